@@ -1,0 +1,136 @@
+"""Benchmark of the state-vector engine on one MI355X.
+
+Default workload (one step = one full application of the circuit to the
+state resident in HBM): the 30-qubit FP64 random circuit, 20 layers of U3 on
+every qubit + brick-wall CZ (890 gates, 16 GiB state).  Prints ONE JSON line:
+value = gates/s; roofline = the tiled sweep kernel's algorithmic bytes
+(one read + one write of the state, 32 * 2^n B per launch) over its average
+launch duration, timed with HIP events on the engine's stream; cpu_baseline =
+the CPU oracle (oracle/, OpenMP) on a bounded prefix of the same circuit.
+
+  python bench.py [--steps K] [--warmup W] [--extra] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "sv-sim_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "gates/s + HBM TB/s (% roofline) at 30–33q FP64; adjoint grad time vs param-shift"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bench_u3_brickwall(n, steps, warmup):
+    import svmi
+    from svmi import workloads as W
+    gates = W.u3_brickwall(n=n, layers=20)
+    e = svmi.Engine(n)
+    e.upload(gates)
+    for _ in range(warmup):
+        e.run()
+    e.set_option(svmi.OPT_PROFILE, 1)
+    sweep_ms, sweep_n, other_ms = 0.0, 0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.run()
+        st = e.stats()
+        sweep_ms += st["sweep_ms"]
+        sweep_n += st["sweep_launches"]
+        other_ms += st["other_ms"]
+    dt = time.perf_counter() - t0
+    e.set_option(svmi.OPT_PROFILE, 0)
+    # untimed: sampling throughput with fed draws
+    _, u = W.draws(4096, seed=2002)
+    t1 = time.perf_counter()
+    _, total = e.measure(u)
+    ts = time.perf_counter() - t1
+    passes = e.stats()["sweeps"]
+    e.close()
+    return {"gates": len(gates), "dt": dt, "sweep_ms": sweep_ms, "sweep_n": sweep_n, "other_ms": other_ms,
+            "passes": passes, "sample_s": ts, "total_prob": total}
+
+
+def cpu_baseline_u3(n, budget_s=12.0, threads=None):
+    """Oracle (OpenMP C restatement of the reference CPU engine) on the first
+    gates of the same circuit, in chunks until ~budget_s of CPU work."""
+    from oracle import oracle as O
+    from svmi import workloads as W
+    threads = threads or min(16, os.cpu_count() or 1)
+    gates = W.u3_brickwall(n=n, layers=20)
+    re, im = O.zero_state(n)
+    done, t = 0, 0.0
+    while t < budget_s and done < len(gates):
+        chunk = gates[done:done + 5]
+        t0 = time.perf_counter()
+        rc = O.lib().orc_apply_gates(n, O._ptr(re), O._ptr(im), O._ptr(np.ascontiguousarray(chunk)), len(chunk), threads)
+        t += time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError("oracle failed")
+        done += len(chunk)
+    return {"value": done / t, "unit": "gates/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of 890 gates of the {n}-qubit u3 brick-wall circuit, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--extra", action="store_true", help="also run the QFT, adjoint and batch workloads")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine runs on a single device)",
+              file=sys.stderr)
+        sys.exit(2)
+    log(f"[bench] u3 brick-wall n={a.qubits} warmup={a.warmup} steps={a.steps}")
+    r = bench_u3_brickwall(a.qubits, a.steps, a.warmup)
+    ms = r["dt"] / a.steps * 1e3
+    value = r["gates"] / (r["dt"] / a.steps)
+    bytes_per_sweep = 32.0 * (1 << a.qubits)
+    avg_sweep_s = r["sweep_ms"] / max(r["sweep_n"], 1) / 1e3
+    achieved = bytes_per_sweep / avg_sweep_s / 1e9
+    log(f"[bench] {ms:.1f} ms/step, {r['passes']} passes/step, sweep avg {avg_sweep_s*1e3:.3f} ms, "
+        f"{achieved:.0f} GB/s, sampling 4096 shots {r['sample_s']*1e3:.1f} ms")
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu baseline (oracle, OpenMP)")
+        cpu = cpu_baseline_u3(a.qubits)
+        log(f"[bench] cpu {cpu['value']:.3f} gates/s on {cpu['cores']} threads")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"u3_brickwall n={a.qubits} layers=20 (890 gates, FP64, state in HBM)",
+                   "qubits": a.qubits, "gates": r["gates"], "passes_per_step": r["passes"],
+                   "sweep_GBps": r["passes"] * bytes_per_sweep / (r["dt"] / a.steps) / 1e9,
+                   "sampling_shots_per_s": 4096 / r["sample_s"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+/*
+ * svmi.h -- C ABI of the MI355X state-vector engine (libsvmi.so).
+ *
+ * The engine replaces the hot path of the reference SV-Sim engines
+ * (svsim/src/svsim_cpu_omp.hpp, svsim/src/svsim_amdgpu_sin.hpp): applying
+ * the reference gate set to a 2^n complex FP64 amplitude array, then
+ * measuring and sampling.  The reference's C++ Simulation class and its
+ * pybind11 module (svsim/src/py_wrapper.cpp) sit on top of these entry points
+ * (sv-sim_amd/csrc/svsim_amd.hpp, sv-sim_amd/csrc/py_wrapper.cpp).
+ *
+ * Conventions (identical to the reference):
+ *   - qubit q is bit q of the amplitude index;
+ *   - gate records use the reference enum OP numbering and slot layout
+ *     (svsim_cpu_omp.hpp:46-52, 103-194): U3(theta,phi,lambda,q),
+ *     U2(phi,lambda,q), U1(lambda,q), RX/RY(theta,q), RZ(phi,q),
+ *     CRX/CRY/CRZ/CU1(lambda,c,t), CU3(theta,phi,lambda,c,t),
+ *     RXX/RYY/RZZ/R(theta,...); controls first;
+ *   - every gate has the reference's exact action, including global phases
+ *     (RZ == U1, RZZ == CX.U1.CX, ...) and the non-unitary R and SRN.
+ * Every call returns an svmi_status; svmi_last_error() describes the last
+ * failure on the calling thread.  A handle owns its device state and one HIP
+ * stream and must not be used from two threads at once.
+ */
+#ifndef SVMI_C_ABI_H_
+#define SVMI_C_ABI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVMI_ABI_VERSION 1
+
+typedef enum svmi_status {
+    SVMI_OK = 0,
+    SVMI_ERR_ARG = 1,        /* null pointer, bad size or option           */
+    SVMI_ERR_QUBIT = 2,      /* qubit index >= n or repeated in one gate   */
+    SVMI_ERR_OP = 3,         /* unknown gate opcode                        */
+    SVMI_ERR_STATE = 4,      /* call out of order (run before upload, ...) */
+    SVMI_ERR_DEVICE = 5,     /* no such device / HIP failure               */
+    SVMI_ERR_OOM = 6,        /* device allocation failed                   */
+    SVMI_ERR_UNSUPPORTED = 7 /* e.g. adjoint through a non-unitary gate    */
+} svmi_status;
+
+/* Reference enum OP (svsim_cpu_omp.hpp:46-52), same values. */
+typedef enum svmi_op {
+    SVMI_U3, SVMI_U2, SVMI_U1, SVMI_CX, SVMI_ID, SVMI_X, SVMI_Y, SVMI_Z, SVMI_H, SVMI_S,
+    SVMI_SDG, SVMI_T, SVMI_TDG, SVMI_RX, SVMI_RY, SVMI_RZ, SVMI_CZ, SVMI_CY, SVMI_SWAP, SVMI_CH,
+    SVMI_CCX, SVMI_CSWAP, SVMI_CRX, SVMI_CRY, SVMI_CRZ, SVMI_CU1, SVMI_CU3, SVMI_RXX, SVMI_RZZ, SVMI_RCCX,
+    SVMI_RC3X, SVMI_C3X, SVMI_C3SQRTX, SVMI_C4X, SVMI_R, SVMI_SRN, SVMI_W, SVMI_RYY,
+    SVMI_NUM_OPS
+} svmi_op;
+
+/* One gate: the reference Gate record (svsim_cpu_omp.hpp:103-194) without
+ * its function pointer.  Unused qubit slots must be valid qubits (the
+ * reference factories set them to 0; append() asserts all five < n). */
+typedef struct svmi_gate {
+    int32_t op;
+    int32_t qb[5];
+    double theta;
+    double phi;
+    double lambda;
+} svmi_gate;
+
+/* Pauli-string term coeff * P with P = i^{|x&z|} X^x Z^z: bit q of xmask
+ * and zmask selects X (x only), Z (z only) or Y (both) on qubit q. */
+typedef struct svmi_pauli_term {
+    double coeff;
+    uint64_t xmask;
+    uint64_t zmask;
+} svmi_pauli_term;
+
+typedef struct svmi_sim* svmi_handle;
+
+/* Execution statistics of the last run (for benchmarks and tests). */
+typedef struct svmi_stats {
+    uint64_t sweeps;         /* full state passes executed on the device      */
+    uint64_t kernel_launches;
+    uint64_t ops;            /* device ops after lowering and fusion          */
+    uint64_t gates;          /* reference gates in the uploaded circuit       */
+    double device_ms;        /* HIP-event time of the last svmi_run           */
+    /* with SVMI_OPT_PROFILE: per-launch HIP-event times of the last run      */
+    uint64_t sweep_launches; /* tiled sweep kernels launched                  */
+    double sweep_ms;         /* summed duration of those launches             */
+    uint64_t other_launches; /* standalone-op kernels                         */
+    double other_ms;
+} svmi_stats;
+
+/* Engine options (svmi_set_option). */
+typedef enum svmi_option {
+    SVMI_OPT_FUSION = 0,      /* 1 = tiled multi-gate sweeps (default), 0 = one pass per op */
+    SVMI_OPT_TILE_QUBITS = 1, /* qubits per LDS tile (6..12, default 12)                    */
+    SVMI_OPT_LOW_QUBITS = 2,  /* low qubits every tile keeps for coalescing (2..6, default 4) */
+    SVMI_OPT_SWEEP_FLOPS = 3, /* per-amplitude FMA budget of one sweep (0 = unlimited)      */
+    SVMI_OPT_PROFILE = 4      /* 1 = time every launch with HIP events (svmi_stats)         */
+} svmi_option;
+
+/* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,
+ *      svsim_cpu_omp.hpp:199-228).  `device` is the HIP ordinal, the second
+ *      constructor argument of the AMD single-GPU engine (i_gpu). */
+svmi_status svmi_create(int n_qubits, int device, svmi_handle* out);
+svmi_status svmi_destroy(svmi_handle h);
+int svmi_num_qubits(svmi_handle h);
+
+/* ---- state: reset_sv (svsim_cpu_omp.hpp:235-240) and the public
+ *      sv_real/sv_imag members (svsim_cpu_omp.hpp:576-577). Host SoA arrays of
+ *      2^n doubles each. */
+svmi_status svmi_reset_state(svmi_handle h);
+svmi_status svmi_set_state(svmi_handle h, const double* re, const double* im);
+svmi_status svmi_get_state(svmi_handle h, double* re, double* im);
+
+/* ---- circuit: append+upload / clear_circuit (svsim_cpu_omp.hpp:242-275,
+ *      321-338).  upload validates (qubit bounds, distinct qubits, opcode),
+ *      lowers and schedules the gate list; the engine keeps its own copy. */
+svmi_status svmi_upload(svmi_handle h, const svmi_gate* gates, size_t n_gates);
+svmi_status svmi_clear_circuit(svmi_handle h);
+
+/* ---- run: Simulation::sim (svsim_cpu_omp.hpp:288-319,
+ *      svsim_amdgpu_sin.hpp:320-353).  Applies the uploaded circuit to the
+ *      current device state; blocks until done.  No host copy is made. */
+svmi_status svmi_run(svmi_handle h);
+
+/* ---- measure: Simulation::measure (svsim_cpu_omp.hpp:339-361).  For each
+ *      draw r = uniforms[i] in [0,1] returns the basis index j with
+ *      scan[j] <= r < scan[j+1] (scan = inclusive prefix sum of |amp|^2),
+ *      or 0 if no interval matches.  The state is not collapsed.  If
+ *      total_prob is non-null it receives scan[2^n]. */
+svmi_status svmi_measure(svmi_handle h, const double* uniforms, size_t shots,
+                         uint64_t* out, double* total_prob);
+
+/* ---- expectation of a Pauli sum (no reference counterpart). */
+svmi_status svmi_expval(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms,
+                        double* out);
+
+/* ---- adjoint differentiation (no reference counterpart).  Applies the
+ *      uploaded circuit to the current state psi0 (the state afterwards is
+ *      U psi0, as after svmi_run), returns E = <H> and
+ *      grad[p] = dE/dtheta_p.  param_slots has 3 entries per uploaded gate
+ *      (theta, phi, lambda slot) holding a parameter index in [0, n_params)
+ *      or -1; a parameter may be shared by several slots.  Fails with
+ *      SVMI_ERR_UNSUPPORTED if the circuit contains R or SRN (not unitary). */
+svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms,
+                         const int32_t* param_slots, size_t n_params,
+                         double* expval_out, double* grad_out);
+
+/* ---- batched expectation + adjoint gradient of many independent circuits,
+ *      each started from |0..0>, in one persistent device launch. */
+typedef struct svmi_batch_item {
+    int32_t n_qubits;
+    int32_t reserved;
+    const svmi_gate* gates;
+    size_t n_gates;
+    const int32_t* param_slots;     /* 3 per gate, -1 = not trainable */
+    size_t n_params;
+    const svmi_pauli_term* terms;
+    size_t n_terms;
+} svmi_batch_item;
+
+svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t n_items,
+                               double* expvals, double* grads /* concatenated n_params */);
+
+/* ---- options, stats, errors */
+svmi_status svmi_set_option(svmi_handle h, int option, int64_t value);
+svmi_status svmi_get_stats(svmi_handle h, svmi_stats* out);
+const char* svmi_last_error(void);
+int svmi_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVMI_C_ABI_H_ */

@@ -1,0 +1,45 @@
+/*
+ * svmi_debug.h -- host-logic checks of libsvmi.so (no reference counterpart).
+ *
+ * These entry points run the engine's lowering and scheduling and then
+ * execute the resulting device program on the host with the same op code the
+ * HIP kernels use (sv-sim_amd/csrc/svmi_exec.hpp).  They exist so the CPU
+ * test suite can check the planner without a GPU; they never touch a device
+ * and are not reachable from svmi_run or the Simulation API, which always
+ * execute on the GPU.
+ */
+#ifndef SVMI_DEBUG_H
+#define SVMI_DEBUG_H
+
+#include "svmi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Plan `gates` for n qubits with the given options and execute the program
+ * on the host state re/im (2^n doubles each, updated in place). */
+svmi_status svmi_debug_emulate(int n, const svmi_gate* gates, size_t n_gates,
+                               int fusion, int tile_qubits, int low_qubits, int64_t sweep_budget,
+                               double* re, double* im);
+
+/* Host execution of the adjoint program: forward from (re0, im0), then
+ * <H> and the gradient exactly as svmi_adjoint computes them. */
+svmi_status svmi_debug_emulate_adjoint(int n, const svmi_gate* gates, size_t n_gates,
+                                       const svmi_pauli_term* terms, size_t n_terms,
+                                       const int32_t* param_slots, size_t n_params,
+                                       int tile_qubits, int low_qubits,
+                                       const double* re0, const double* im0,
+                                       double* expval_out, double* grad_out);
+
+/* Planner summary: number of full-state passes and a text description
+ * (truncated to buflen-1 bytes; buf may be NULL). */
+svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates,
+                            int fusion, int tile_qubits, int low_qubits, int64_t sweep_budget,
+                            uint64_t* n_passes, char* buf, size_t buflen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVMI_DEBUG_H */

@@ -1,0 +1,890 @@
+// svmi_api.hip -- C ABI of the engine (include/svmi.h, include/svmi_debug.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/svmi.h"
+#include "../../include/svmi_debug.h"
+#include "svmi_exec.hpp"
+#include "svmi_kernels.hpp"
+#include "svmi_lower.hpp"
+#include "svmi_plan.hpp"
+
+using namespace svmi;
+
+namespace {
+
+thread_local std::string g_err;
+
+svmi_status fail(svmi_status st, const std::string& msg)
+{
+    g_err = msg;
+    return st;
+}
+
+#define SVMI_HIP(x)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? SVMI_ERR_OOM : SVMI_ERR_DEVICE,          \
+                        std::string(#x) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+#define SVMI_TRY(x)                        \
+    do {                                   \
+        svmi_status s_ = (x);              \
+        if (s_ != SVMI_OK) return s_;      \
+    } while (0)
+
+constexpr int kMaxQubits = 40;
+
+// Growable device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    svmi_status ensure(size_t want)
+    {
+        if (want <= bytes) return SVMI_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        SVMI_HIP(hipMalloc(&p, want));
+        bytes = want;
+        return SVMI_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct DeviceProgram {
+    DevBuf phases, ops, coef, hitab;
+    template <class T>
+    static svmi_status put(DevBuf& b, const std::vector<T>& v, hipStream_t st)
+    {
+        const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+        SVMI_TRY(b.ensure(bytes));
+        if (!v.empty()) SVMI_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        return SVMI_OK;
+    }
+    svmi_status upload(const Program& p, hipStream_t st)
+    {
+        SVMI_TRY(put(phases, p.phases, st));
+        SVMI_TRY(put(ops, p.ops, st));
+        SVMI_TRY(put(coef, p.coef, st));
+        SVMI_TRY(put(hitab, p.hitab, st));
+        SVMI_HIP(hipStreamSynchronize(st));
+        return SVMI_OK;
+    }
+    void release()
+    {
+        phases.release();
+        ops.release();
+        coef.release();
+        hitab.release();
+    }
+};
+
+svmi_status check_gates(const svmi_gate* gates, size_t n_gates, int n)
+{
+    if (n_gates && !gates) return fail(SVMI_ERR_ARG, "null gate array");
+    for (size_t i = 0; i < n_gates; i++) {
+        std::string err;
+        const svmi_status st = validate_gate(gates[i], n, err);
+        if (st != SVMI_OK) return fail(st, "gate " + std::to_string(i) + ": " + err);
+    }
+    return SVMI_OK;
+}
+
+void lower_all(const svmi_gate* gates, size_t n_gates, std::vector<LOp>& ops)
+{
+    ops.resize(n_gates);
+    for (size_t i = 0; i < n_gates; i++) {
+        lower_gate(gates[i], ops[i]);
+        ops[i].gate = int(i);
+    }
+}
+
+void plan_forward(const svmi_gate* gates, size_t n_gates, int n, const PlanOptions& opt, Program& prog)
+{
+    std::vector<LOp> low, fused;
+    lower_all(gates, n_gates, low);
+    if (opt.fusion) {
+        fuse_single_qubit_runs(low, fused);
+        build_forward_program(fused, n, opt, prog);
+    } else {
+        build_forward_program(low, n, opt, prog);
+    }
+}
+
+svmi_status check_terms(const svmi_pauli_term* terms, size_t n_terms, int n)
+{
+    if (n_terms && !terms) return fail(SVMI_ERR_ARG, "null term array");
+    const uint64_t lim = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    for (size_t i = 0; i < n_terms; i++)
+        if ((terms[i].xmask | terms[i].zmask) & ~lim)
+            return fail(SVMI_ERR_QUBIT, "Pauli term " + std::to_string(i) + " acts on a qubit >= n");
+    return SVMI_OK;
+}
+
+// Group terms by X mask; coefficients absorb i^{|x&z|}.
+void group_terms(const svmi_pauli_term* terms, size_t n_terms, std::vector<PauliGroup>& groups,
+                 std::vector<PauliTerm>& out)
+{
+    std::vector<size_t> idx(n_terms);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return terms[a].xmask < terms[b].xmask; });
+    groups.clear();
+    out.clear();
+    for (size_t i : idx) {
+        const svmi_pauli_term& t = terms[i];
+        if (groups.empty() || groups.back().xmask != t.xmask)
+            groups.push_back(PauliGroup{t.xmask, uint32_t(out.size()), uint32_t(out.size())});
+        const int ny = __builtin_popcountll(t.xmask & t.zmask) & 3;
+        const double re[4] = {1, 0, -1, 0}, im[4] = {0, 1, 0, -1};
+        out.push_back(PauliTerm{t.zmask, t.coeff * re[ny], t.coeff * im[ny]});
+        groups.back().end = uint32_t(out.size());
+    }
+}
+
+svmi_status check_params(const svmi_gate* gates, size_t n_gates, const int32_t* param_slots, size_t n_params)
+{
+    if (n_gates && !param_slots) return fail(SVMI_ERR_ARG, "null param_slots");
+    for (size_t i = 0; i < n_gates; i++) {
+        if (!gate_is_unitary(gates[i].op))
+            return fail(SVMI_ERR_UNSUPPORTED, "adjoint needs unitary gates; gate " + std::to_string(i) +
+                                                  " (R or SRN) is not unitary");
+        for (int s = 0; s < 3; s++) {
+            const int32_t p = param_slots[3 * i + s];
+            if (p < -1 || (p >= 0 && size_t(p) >= n_params))
+                return fail(SVMI_ERR_ARG, "param_slots[" + std::to_string(3 * i + s) + "] out of range");
+        }
+    }
+    return SVMI_OK;
+}
+
+// Backward entries: for gates in reverse, U^dagger on (phi, lambda), then
+// K = U^dagger dU/dtheta_s for each trainable slot (read only).
+void adjoint_entries(const svmi_gate* gates, size_t n_gates, const int32_t* param_slots,
+                     std::vector<AdjEntry>& entries)
+{
+    entries.clear();
+    for (size_t ii = n_gates; ii-- > 0;) {
+        LOp U;
+        lower_gate(gates[ii], U);
+        U.gate = int(ii);
+        AdjEntry inv;
+        lop_adjoint(U, inv.op);
+        entries.push_back(inv);
+        for (int s = 0; s < 3; s++) {
+            const int32_t p = param_slots[3 * ii + s];
+            if (p < 0) continue;
+            LOp dU;
+            if (!gate_derivative(gates[ii], s, dU)) continue;
+            dU.gate = int(ii);
+            AdjEntry g;
+            g.grad = true;
+            g.param = p;
+            // K = U^dagger * dU  (dU applied first, then U^dagger)
+            if (inv.op.kind == LOp::DIAG && dU.kind == LOp::DIAG) {
+                g.op = dU;
+                for (size_t r = 0; r < dU.m.size(); r++) g.op.m[r] = cmul(inv.op.m[r], dU.m[r]);
+            } else {
+                lop_compose_same(dU, inv.op, g.op);
+            }
+            entries.push_back(g);
+        }
+    }
+}
+
+}  // namespace
+
+struct svmi_sim {
+    int n = 0;          // qubits seen by the caller
+    int n_int = 0;      // qubits of the device state (>= 4)
+    int device = 0;
+    uint64_t dim = 0;   // 2^n_int
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    c64* sv = nullptr;
+    DevBuf aux[2];
+    DevBuf scratch[6];
+    std::vector<svmi_gate> gates;
+    bool uploaded = false;
+    Program prog;
+    DeviceProgram dprog;
+    PlanOptions opt;
+    bool profile = false;
+    std::vector<hipEvent_t> prof_events;
+    svmi_stats stats{};
+};
+
+namespace {
+
+svmi_status use_device(svmi_handle h)
+{
+    if (!h) return fail(SVMI_ERR_ARG, "null handle");
+    SVMI_HIP(hipSetDevice(h->device));
+    return SVMI_OK;
+}
+
+svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, c64* s0, c64* s1,
+                         double* grad_sums /* device, prog.n_grad */)
+{
+    const bool prof = h->profile;
+    if (prof) {
+        while (h->prof_events.size() < 2 * prog.steps.size()) {
+            hipEvent_t e;
+            SVMI_HIP(hipEventCreate(&e));
+            h->prof_events.push_back(e);
+        }
+    }
+    size_t si = 0;
+    for (const Step& st : prog.steps) {
+        if (prof) SVMI_HIP(hipEventRecord(h->prof_events[2 * si], h->stream));
+        if (st.kind == Step::SWEEP) {
+            const SweepInfo& sw = prog.sweeps[st.index];
+            double* partials = nullptr;
+            if (prog.n_states == 2 && sw.n_grad) {
+                SVMI_TRY(h->scratch[5].ensure(sw.n_tiles * sw.n_grad * sizeof(double)));
+                partials = h->scratch[5].as<double>();
+            }
+            SVMI_HIP(launch_sweep(h->stream, prog.n_states, s0, s1, sw, dp.phases.as<PhaseInfo>(),
+                                  dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials));
+            if (partials)
+                SVMI_HIP(launch_reduce_grads(h->stream, partials, sw.n_tiles, sw.n_grad, grad_sums + sw.grad_base));
+        } else {
+            const StandaloneOp& so = prog.standalone[st.index];
+            SVMI_HIP(launch_standalone(h->stream, s0, h->n_int, so, dp.coef.as<c64>()));
+            if (prog.n_states == 2) SVMI_HIP(launch_standalone(h->stream, s1, h->n_int, so, dp.coef.as<c64>()));
+        }
+        if (prof) SVMI_HIP(hipEventRecord(h->prof_events[2 * si + 1], h->stream));
+        h->stats.kernel_launches++;
+        si++;
+    }
+    h->stats.sweeps += prog.steps.size();
+    if (prof) {
+        SVMI_HIP(hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < prog.steps.size(); i++) {
+            float ms = 0.f;
+            SVMI_HIP(hipEventElapsedTime(&ms, h->prof_events[2 * i], h->prof_events[2 * i + 1]));
+            if (prog.steps[i].kind == Step::SWEEP) {
+                h->stats.sweep_launches++;
+                h->stats.sweep_ms += ms;
+            } else {
+                h->stats.other_launches++;
+                h->stats.other_ms += ms;
+            }
+        }
+    }
+    return SVMI_OK;
+}
+
+// <psi|H|psi>, optionally writing lam = H psi.
+svmi_status pauli_device(svmi_handle h, const c64* psi, c64* lam, const svmi_pauli_term* terms, size_t n_terms,
+                         double* out)
+{
+    std::vector<PauliGroup> groups;
+    std::vector<PauliTerm> pt;
+    group_terms(terms, n_terms, groups, pt);
+    const uint32_t grid = pauli_grid(h->dim);
+    SVMI_TRY(h->scratch[0].ensure(std::max<size_t>(groups.size(), 1) * sizeof(PauliGroup)));
+    SVMI_TRY(h->scratch[1].ensure(std::max<size_t>(pt.size(), 1) * sizeof(PauliTerm)));
+    SVMI_TRY(h->scratch[2].ensure(size_t(grid) * sizeof(double)));
+    if (!groups.empty()) {
+        SVMI_HIP(hipMemcpyAsync(h->scratch[0].p, groups.data(), groups.size() * sizeof(PauliGroup),
+                                hipMemcpyHostToDevice, h->stream));
+        SVMI_HIP(hipMemcpyAsync(h->scratch[1].p, pt.data(), pt.size() * sizeof(PauliTerm), hipMemcpyHostToDevice,
+                                h->stream));
+    }
+    SVMI_HIP(launch_pauli(h->stream, psi, lam, h->dim, h->scratch[0].as<PauliGroup>(), uint32_t(groups.size()),
+                          h->scratch[1].as<PauliTerm>(), h->scratch[2].as<double>()));
+    std::vector<double> part(grid);
+    SVMI_HIP(hipMemcpyAsync(part.data(), h->scratch[2].p, grid * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *out = s;
+    return SVMI_OK;
+}
+
+// Host copies in chunks (AoS device <-> SoA host).
+constexpr size_t kChunk = size_t(1) << 22;  // amplitudes per staging chunk (64 MiB)
+
+}  // namespace
+
+extern "C" {
+
+int svmi_abi_version(void) { return SVMI_ABI_VERSION; }
+
+const char* svmi_last_error(void) { return g_err.c_str(); }
+
+svmi_status svmi_create(int n_qubits, int device, svmi_handle* out)
+{
+    if (!out) return fail(SVMI_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (n_qubits < 1 || n_qubits > kMaxQubits)
+        return fail(SVMI_ERR_ARG, "n_qubits must be in [1, " + std::to_string(kMaxQubits) + "]");
+    int ndev = 0;
+    SVMI_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(SVMI_ERR_DEVICE, "device " + std::to_string(device) + " not present (" + std::to_string(ndev) +
+                                         " visible)");
+    SVMI_HIP(hipSetDevice(device));
+    static bool kernels_ready = false;
+    if (!kernels_ready) {
+        SVMI_HIP(init_sweep_kernels());
+        kernels_ready = true;
+    }
+    svmi_sim* h = new svmi_sim();
+    h->n = n_qubits;
+    h->n_int = std::max(n_qubits, kRegBits);
+    h->device = device;
+    h->dim = 1ull << h->n_int;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->sv), h->dim * sizeof(c64));
+    if (e == hipSuccess) e = launch_fill_basis(h->stream, h->sv, h->dim);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        const std::string msg = std::string("create: ") + hipGetErrorString(e);
+        svmi_destroy(h);
+        return fail(e == hipErrorOutOfMemory ? SVMI_ERR_OOM : SVMI_ERR_DEVICE, msg);
+    }
+    *out = h;
+    return SVMI_OK;
+}
+
+svmi_status svmi_destroy(svmi_handle h)
+{
+    if (!h) return SVMI_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->sv) (void)hipFree(h->sv);
+    for (auto& b : h->aux) b.release();
+    for (auto& b : h->scratch) b.release();
+    for (hipEvent_t e : h->prof_events) (void)hipEventDestroy(e);
+    h->dprog.release();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SVMI_OK;
+}
+
+int svmi_num_qubits(svmi_handle h) { return h ? h->n : -1; }
+
+svmi_status svmi_reset_state(svmi_handle h)
+{
+    SVMI_TRY(use_device(h));
+    SVMI_HIP(launch_fill_basis(h->stream, h->sv, h->dim));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    return SVMI_OK;
+}
+
+svmi_status svmi_set_state(svmi_handle h, const double* re, const double* im)
+{
+    SVMI_TRY(use_device(h));
+    if (!re || !im) return fail(SVMI_ERR_ARG, "null state array");
+    const uint64_t dim_user = 1ull << h->n;
+    if (h->n_int != h->n) {
+        SVMI_HIP(launch_fill_basis(h->stream, h->sv, h->dim));
+    }
+    c64* stage = nullptr;
+    const size_t cs = std::min<uint64_t>(kChunk, dim_user);
+    SVMI_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage), cs * sizeof(c64), hipHostMallocDefault));
+    svmi_status st = SVMI_OK;
+    for (uint64_t x0 = 0; x0 < dim_user && st == SVMI_OK; x0 += cs) {
+        const size_t cnt = std::min<uint64_t>(cs, dim_user - x0);
+        hipError_t e = hipStreamSynchronize(h->stream);
+        for (size_t i = 0; i < cnt; i++) stage[i] = c64{re[x0 + i], im[x0 + i]};
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h->sv + x0, stage, cnt * sizeof(c64), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) st = fail(SVMI_ERR_DEVICE, std::string("set_state: ") + hipGetErrorString(e));
+    }
+    (void)hipHostFree(stage);
+    return st;
+}
+
+svmi_status svmi_get_state(svmi_handle h, double* re, double* im)
+{
+    SVMI_TRY(use_device(h));
+    if (!re || !im) return fail(SVMI_ERR_ARG, "null state array");
+    const uint64_t dim_user = 1ull << h->n;
+    c64* stage = nullptr;
+    const size_t cs = std::min<uint64_t>(kChunk, dim_user);
+    SVMI_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage), cs * sizeof(c64), hipHostMallocDefault));
+    svmi_status st = SVMI_OK;
+    for (uint64_t x0 = 0; x0 < dim_user && st == SVMI_OK; x0 += cs) {
+        const size_t cnt = std::min<uint64_t>(cs, dim_user - x0);
+        hipError_t e = hipMemcpyAsync(stage, h->sv + x0, cnt * sizeof(c64), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) {
+            st = fail(SVMI_ERR_DEVICE, std::string("get_state: ") + hipGetErrorString(e));
+            break;
+        }
+        for (size_t i = 0; i < cnt; i++) {
+            re[x0 + i] = stage[i].re;
+            im[x0 + i] = stage[i].im;
+        }
+    }
+    (void)hipHostFree(stage);
+    return st;
+}
+
+svmi_status svmi_upload(svmi_handle h, const svmi_gate* gates, size_t n_gates)
+{
+    SVMI_TRY(use_device(h));
+    SVMI_TRY(check_gates(gates, n_gates, h->n));
+    h->gates.assign(gates, gates + n_gates);
+    plan_forward(gates, n_gates, h->n_int, h->opt, h->prog);
+    SVMI_TRY(h->dprog.upload(h->prog, h->stream));
+    h->uploaded = true;
+    h->stats.gates = n_gates;
+    h->stats.ops = h->prog.n_lowered;
+    return SVMI_OK;
+}
+
+svmi_status svmi_clear_circuit(svmi_handle h)
+{
+    if (!h) return fail(SVMI_ERR_ARG, "null handle");
+    h->gates.clear();
+    h->prog = Program();
+    h->uploaded = false;
+    return SVMI_OK;
+}
+
+svmi_status svmi_run(svmi_handle h)
+{
+    SVMI_TRY(use_device(h));
+    if (!h->uploaded) return fail(SVMI_ERR_STATE, "run() before upload()");
+    h->stats.sweeps = 0;
+    h->stats.kernel_launches = 0;
+    h->stats.sweep_launches = h->stats.other_launches = 0;
+    h->stats.sweep_ms = h->stats.other_ms = 0.0;
+    SVMI_HIP(hipEventRecord(h->ev0, h->stream));
+    SVMI_TRY(exec_program(h, h->prog, h->dprog, h->sv, nullptr, nullptr));
+    SVMI_HIP(hipEventRecord(h->ev1, h->stream));
+    SVMI_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    SVMI_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->stats.device_ms = ms;
+    return SVMI_OK;
+}
+
+svmi_status svmi_measure(svmi_handle h, const double* uniforms, size_t shots, uint64_t* out, double* total_prob)
+{
+    SVMI_TRY(use_device(h));
+    if (shots && (!uniforms || !out)) return fail(SVMI_ERR_ARG, "null draws/output");
+    const int s = std::min(h->n_int, 14);
+    const uint64_t bsize = 1ull << s;
+    const uint32_t nthr = uint32_t(std::min<uint64_t>(256, bsize));
+    const uint32_t chunk = uint32_t(bsize / nthr);
+    const uint64_t nb = h->dim >> s;
+    SVMI_TRY(h->scratch[3].ensure(nb * sizeof(double)));
+    SVMI_HIP(launch_block_totals(h->stream, h->sv, nb, nthr, chunk, h->scratch[3].as<double>()));
+    std::vector<double> tot(nb);
+    SVMI_HIP(hipMemcpyAsync(tot.data(), h->scratch[3].p, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    std::vector<double> O(nb + 1);
+    O[0] = 0.0;
+    for (uint64_t b = 0; b < nb; b++) O[b + 1] = O[b] + tot[b];
+    if (total_prob) *total_prob = O[nb];
+    if (shots == 0) return SVMI_OK;
+    // block of each shot (-1: no interval matches -> result 0)
+    std::vector<int64_t> blk(shots);
+    for (size_t i = 0; i < shots; i++) {
+        const double r = uniforms[i];
+        if (!(r >= O[0]) || !(r < O[nb])) {
+            blk[i] = -1;
+            continue;
+        }
+        const uint64_t m = uint64_t(std::upper_bound(O.begin(), O.end(), r) - O.begin());  // first O[m] > r
+        blk[i] = int64_t(m) - 1;
+    }
+    std::vector<size_t> order;
+    order.reserve(shots);
+    for (size_t i = 0; i < shots; i++)
+        if (blk[i] >= 0) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return blk[a] < blk[b]; });
+    std::vector<uint64_t> jobs;
+    std::vector<double> offs, draws(order.size());
+    for (size_t j = 0; j < order.size(); j++) {
+        draws[j] = uniforms[order[j]];
+        const uint64_t b = uint64_t(blk[order[j]]);
+        if (jobs.empty() || jobs[jobs.size() - 3] != b) {
+            jobs.push_back(b);
+            jobs.push_back(j);
+            jobs.push_back(0);
+            offs.push_back(O[b]);
+        }
+        jobs.back()++;
+    }
+    const uint32_t njobs = uint32_t(offs.size());
+    std::vector<uint64_t> res(order.size());
+    if (njobs) {
+        SVMI_TRY(h->scratch[0].ensure(jobs.size() * sizeof(uint64_t)));
+        SVMI_TRY(h->scratch[1].ensure(offs.size() * sizeof(double)));
+        SVMI_TRY(h->scratch[2].ensure(draws.size() * sizeof(double)));
+        SVMI_TRY(h->scratch[4].ensure(res.size() * sizeof(uint64_t)));
+        SVMI_HIP(hipMemcpyAsync(h->scratch[0].p, jobs.data(), jobs.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                h->stream));
+        SVMI_HIP(hipMemcpyAsync(h->scratch[1].p, offs.data(), offs.size() * sizeof(double), hipMemcpyHostToDevice,
+                                h->stream));
+        SVMI_HIP(hipMemcpyAsync(h->scratch[2].p, draws.data(), draws.size() * sizeof(double), hipMemcpyHostToDevice,
+                                h->stream));
+        SVMI_HIP(launch_sample(h->stream, h->sv, nthr, chunk, h->scratch[0].as<uint64_t>(), njobs,
+                               h->scratch[1].as<double>(), h->scratch[2].as<double>(), h->scratch[4].as<uint64_t>()));
+        SVMI_HIP(hipMemcpyAsync(res.data(), h->scratch[4].p, res.size() * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                h->stream));
+        SVMI_HIP(hipStreamSynchronize(h->stream));
+    }
+    for (size_t i = 0; i < shots; i++) out[i] = 0;
+    for (size_t j = 0; j < order.size(); j++) out[order[j]] = res[j];
+    return SVMI_OK;
+}
+
+svmi_status svmi_expval(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms, double* out)
+{
+    SVMI_TRY(use_device(h));
+    if (!out) return fail(SVMI_ERR_ARG, "null output");
+    SVMI_TRY(check_terms(terms, n_terms, h->n));
+    return pauli_device(h, h->sv, nullptr, terms, n_terms, out);
+}
+
+svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms, const int32_t* param_slots,
+                         size_t n_params, double* expval_out, double* grad_out)
+{
+    SVMI_TRY(use_device(h));
+    if (!h->uploaded) return fail(SVMI_ERR_STATE, "adjoint() before upload()");
+    if (!expval_out || (n_params && !grad_out)) return fail(SVMI_ERR_ARG, "null output");
+    SVMI_TRY(check_terms(terms, n_terms, h->n));
+    SVMI_TRY(check_params(h->gates.data(), h->gates.size(), param_slots, n_params));
+    // forward
+    SVMI_TRY(svmi_run(h));
+    // phi = psi, lam = H psi, E = <psi|lam>
+    SVMI_TRY(h->aux[0].ensure(h->dim * sizeof(c64)));
+    SVMI_TRY(h->aux[1].ensure(h->dim * sizeof(c64)));
+    c64* phi = h->aux[0].as<c64>();
+    c64* lam = h->aux[1].as<c64>();
+    SVMI_HIP(launch_copy(h->stream, phi, h->sv, h->dim));
+    double E = 0.0;
+    SVMI_TRY(pauli_device(h, h->sv, lam, terms, n_terms, &E));
+    *expval_out = E;
+    for (size_t p = 0; p < n_params; p++) grad_out[p] = 0.0;
+    // backward
+    std::vector<AdjEntry> entries;
+    adjoint_entries(h->gates.data(), h->gates.size(), param_slots, entries);
+    Program bp;
+    build_adjoint_program(entries, h->n_int, h->opt, bp);
+    DeviceProgram dbp;
+    svmi_status st = dbp.upload(bp, h->stream);
+    DevBuf gsum;
+    if (st == SVMI_OK) st = gsum.ensure(std::max<size_t>(bp.n_grad, 1) * sizeof(double));
+    if (st == SVMI_OK) st = exec_program(h, bp, dbp, phi, lam, gsum.as<double>());
+    std::vector<double> g(bp.n_grad);
+    if (st == SVMI_OK && bp.n_grad) {
+        hipError_t e = hipMemcpyAsync(g.data(), gsum.p, bp.n_grad * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) st = fail(SVMI_ERR_DEVICE, std::string("adjoint: ") + hipGetErrorString(e));
+    }
+    if (st == SVMI_OK) {
+        (void)hipStreamSynchronize(h->stream);
+        for (uint32_t i = 0; i < bp.n_grad; i++) grad_out[bp.grad_param[i]] += 2.0 * g[i];
+    }
+    dbp.release();
+    gsum.release();
+    return st;
+}
+
+svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t n_items, double* expvals,
+                               double* grads)
+{
+    if (n_items && (!items || !expvals)) return fail(SVMI_ERR_ARG, "null batch arrays");
+    size_t goff = 0;
+    for (size_t i = 0; i < n_items; i++) {
+        const svmi_batch_item& it = items[i];
+        svmi_handle h = nullptr;
+        SVMI_TRY(svmi_create(it.n_qubits, device, &h));
+        svmi_status st = svmi_upload(h, it.gates, it.n_gates);
+        if (st == SVMI_OK)
+            st = svmi_adjoint(h, it.terms, it.n_terms, it.param_slots, it.n_params, &expvals[i],
+                              grads ? grads + goff : nullptr);
+        svmi_destroy(h);
+        if (st != SVMI_OK) return st;
+        goff += it.n_params;
+    }
+    return SVMI_OK;
+}
+
+svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
+{
+    if (!h) return fail(SVMI_ERR_ARG, "null handle");
+    switch (option) {
+    case SVMI_OPT_FUSION: h->opt.fusion = value != 0; break;
+    case SVMI_OPT_TILE_QUBITS:
+        if (value < kRegBits || value > kMaxTileQubits) return fail(SVMI_ERR_ARG, "tile qubits must be in [4, 12]");
+        h->opt.tile_qubits = int(value);
+        break;
+    case SVMI_OPT_LOW_QUBITS:
+        if (value < 0 || value > 8) return fail(SVMI_ERR_ARG, "low qubits must be in [0, 8]");
+        h->opt.low_qubits = int(value);
+        break;
+    case SVMI_OPT_SWEEP_FLOPS:
+        if (value < 0) return fail(SVMI_ERR_ARG, "sweep budget must be >= 0");
+        h->opt.sweep_budget = long(value);
+        break;
+    case SVMI_OPT_PROFILE: h->profile = value != 0; return SVMI_OK;
+    default: return fail(SVMI_ERR_ARG, "unknown option " + std::to_string(option));
+    }
+    if (h->uploaded) {
+        plan_forward(h->gates.data(), h->gates.size(), h->n_int, h->opt, h->prog);
+        SVMI_TRY(use_device(h));
+        SVMI_TRY(h->dprog.upload(h->prog, h->stream));
+        h->stats.ops = h->prog.n_lowered;
+    }
+    return SVMI_OK;
+}
+
+svmi_status svmi_get_stats(svmi_handle h, svmi_stats* out)
+{
+    if (!h || !out) return fail(SVMI_ERR_ARG, "null argument");
+    *out = h->stats;
+    return SVMI_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================
+// Host execution of device programs (include/svmi_debug.h).
+// =====================================================================
+
+namespace {
+
+void host_standalone(std::vector<c64>& s, int n, const StandaloneOp& so, const std::vector<c64>& coef)
+{
+    const uint64_t dim = 1ull << n;
+    if (so.kind == 2) {
+        const int q = so.qubits[0];
+        for (uint64_t i = 0; i < dim / 2; i++) {
+            const uint64_t p0 = ((i >> q) << (q + 1)) | (i & ((1ull << q) - 1));
+            const uint64_t p1 = p0 | (1ull << q);
+            const c64 a0 = s[p0], a1 = s[p1];
+            s[p0] = c64{0.5 * (a0.re + a1.re), 0.5 * (a0.im - a1.im)};
+            s[p1] = c64{0.5 * (a0.re + a1.re), 0.5 * (-a0.im + a1.im)};
+        }
+        return;
+    }
+    if (so.kind == 1) {
+        for (uint64_t x = 0; x < dim; x++) {
+            uint32_t idx = 0;
+            for (int i = 0; i < so.q; i++) idx |= uint32_t((x >> so.qubits[i]) & 1) << i;
+            s[x] = cmul(s[x], coef[so.coef + idx]);
+        }
+        return;
+    }
+    const uint32_t D = 1u << so.q;
+    uint64_t qm = 0;
+    for (int i = 0; i < so.q; i++) qm |= 1ull << so.qubits[i];
+    std::vector<c64> v(D);
+    for (uint64_t b = 0; b < dim; b++) {
+        if (b & qm) continue;
+        for (uint32_t j = 0; j < D; j++) {
+            uint64_t o = b;
+            for (int i = 0; i < so.q; i++)
+                if ((j >> i) & 1) o |= 1ull << so.qubits[i];
+            v[j] = s[o];
+        }
+        for (uint32_t r = 0; r < D; r++) {
+            c64 acc{0.0, 0.0};
+            for (uint32_t j = 0; j < D; j++) acc = cmac(acc, coef[so.coef + r * D + j], v[j]);
+            uint64_t o = b;
+            for (int i = 0; i < so.q; i++)
+                if ((r >> i) & 1) o |= 1ull << so.qubits[i];
+            s[o] = acc;
+        }
+    }
+}
+
+void host_program(const Program& prog, std::vector<c64>& s0, std::vector<c64>& s1, std::vector<double>& gsum)
+{
+    const int ns = prog.n_states;
+    gsum.assign(prog.n_grad, 0.0);
+    for (const Step& st : prog.steps) {
+        if (st.kind == Step::STANDALONE) {
+            host_standalone(s0, prog.n, prog.standalone[st.index], prog.coef);
+            if (ns == 2) host_standalone(s1, prog.n, prog.standalone[st.index], prog.coef);
+            continue;
+        }
+        const SweepInfo& sw = prog.sweeps[st.index];
+        const uint32_t tsize = 1u << sw.k, ngroups = tsize >> kRegBits;
+        const uint32_t cmask = (1u << sw.c) - 1u;
+        std::vector<c64> t0(tsize), t1(tsize);
+        std::vector<uint64_t> gidx(tsize);
+        for (uint64_t tile = 0; tile < sw.n_tiles; tile++) {
+            const uint64_t base = tile_base(sw, tile);
+            for (uint32_t l = 0; l < tsize; l++) {
+                gidx[l] = base | prog.hitab[sw.hitab + (l >> sw.c)] | (l & cmask);
+                t0[l] = s0[gidx[l]];
+                if (ns == 2) t1[l] = s1[gidx[l]];
+            }
+            for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
+                const PhaseInfo& ph = prog.phases[p];
+                uint32_t off[kRegs];
+                phase_offsets(ph, off);
+                for (uint32_t g = 0; g < ngroups; g++) {
+                    const uint32_t goff = group_offset(ph, g);
+                    c64 a[kRegs], b[kRegs];
+                    for (int r = 0; r < kRegs; r++) {
+                        a[r] = t0[goff | off[r]];
+                        b[r] = ns == 2 ? t1[goff | off[r]] : c64{0.0, 0.0};
+                    }
+                    for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
+                        const DOp& op = prog.ops[o];
+                        if (op.type == DOP_GRAD_DENSE || op.type == DOP_GRAD_DIAG) {
+                            gsum[sw.grad_base + op.grad] += exec_grad_op(op, a, b, prog.coef.data(), goff, off, base);
+                        } else {
+                            exec_state_op(op, a, prog.coef.data(), goff, off, base);
+                            if (ns == 2) exec_state_op(op, b, prog.coef.data(), goff, off, base);
+                        }
+                    }
+                    for (int r = 0; r < kRegs; r++) {
+                        t0[goff | off[r]] = a[r];
+                        if (ns == 2) t1[goff | off[r]] = b[r];
+                    }
+                }
+            }
+            for (uint32_t l = 0; l < tsize; l++) {
+                s0[gidx[l]] = t0[l];
+                if (ns == 2) s1[gidx[l]] = t1[l];
+            }
+        }
+    }
+}
+
+void host_pauli(int n, const std::vector<c64>& psi, std::vector<c64>* lam, const svmi_pauli_term* terms,
+                size_t n_terms, double* E)
+{
+    std::vector<PauliGroup> groups;
+    std::vector<PauliTerm> pt;
+    group_terms(terms, n_terms, groups, pt);
+    const uint64_t dim = 1ull << n;
+    double acc = 0.0;
+    for (uint64_t y = 0; y < dim; y++) {
+        c64 l{0.0, 0.0};
+        for (const PauliGroup& G : groups) {
+            const uint64_t x = y ^ G.xmask;
+            double fr = 0.0, fi = 0.0;
+            for (uint32_t t = G.begin; t < G.end; t++) {
+                const bool neg = __builtin_popcountll(x & pt[t].zmask) & 1;
+                fr += neg ? -pt[t].cre : pt[t].cre;
+                fi += neg ? -pt[t].cim : pt[t].cim;
+            }
+            l = cmac(l, c64{fr, fi}, psi[x]);
+        }
+        if (lam) (*lam)[y] = l;
+        acc += psi[y].re * l.re + psi[y].im * l.im;
+    }
+    *E = acc;
+}
+
+PlanOptions debug_options(int fusion, int tile_qubits, int low_qubits, int64_t budget)
+{
+    PlanOptions o;
+    o.fusion = fusion != 0;
+    o.tile_qubits = tile_qubits;
+    o.low_qubits = low_qubits;
+    o.sweep_budget = long(budget);
+    return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+svmi_status svmi_debug_emulate(int n, const svmi_gate* gates, size_t n_gates, int fusion, int tile_qubits,
+                               int low_qubits, int64_t sweep_budget, double* re, double* im)
+{
+    if (n < 1 || n > 26) return fail(SVMI_ERR_ARG, "emulation supports 1..26 qubits");
+    if (!re || !im) return fail(SVMI_ERR_ARG, "null state");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    SVMI_TRY(check_gates(gates, n_gates, n));
+    const int ni = std::max(n, kRegBits);
+    Program prog;
+    plan_forward(gates, n_gates, ni, debug_options(fusion, tile_qubits, low_qubits, sweep_budget), prog);
+    std::vector<c64> s(1ull << ni, c64{0.0, 0.0}), dummy;
+    for (uint64_t x = 0; x < (1ull << n); x++) s[x] = c64{re[x], im[x]};
+    std::vector<double> g;
+    host_program(prog, s, dummy, g);
+    for (uint64_t x = 0; x < (1ull << n); x++) {
+        re[x] = s[x].re;
+        im[x] = s[x].im;
+    }
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_emulate_adjoint(int n, const svmi_gate* gates, size_t n_gates, const svmi_pauli_term* terms,
+                                       size_t n_terms, const int32_t* param_slots, size_t n_params, int tile_qubits,
+                                       int low_qubits, const double* re0, const double* im0, double* expval_out,
+                                       double* grad_out)
+{
+    if (n < 1 || n > 22) return fail(SVMI_ERR_ARG, "emulation supports 1..22 qubits");
+    if (!re0 || !im0 || !expval_out || (n_params && !grad_out)) return fail(SVMI_ERR_ARG, "null argument");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    SVMI_TRY(check_gates(gates, n_gates, n));
+    SVMI_TRY(check_terms(terms, n_terms, n));
+    SVMI_TRY(check_params(gates, n_gates, param_slots, n_params));
+    const int ni = std::max(n, kRegBits);
+    const PlanOptions opt = debug_options(1, tile_qubits, low_qubits, 0);
+    Program fp;
+    plan_forward(gates, n_gates, ni, opt, fp);
+    std::vector<c64> psi(1ull << ni, c64{0.0, 0.0}), dummy;
+    for (uint64_t x = 0; x < (1ull << n); x++) psi[x] = c64{re0[x], im0[x]};
+    std::vector<double> g;
+    host_program(fp, psi, dummy, g);
+    std::vector<c64> lam(psi.size());
+    double E = 0.0;
+    host_pauli(ni, psi, &lam, terms, n_terms, &E);
+    *expval_out = E;
+    std::vector<AdjEntry> entries;
+    adjoint_entries(gates, n_gates, param_slots, entries);
+    Program bp;
+    build_adjoint_program(entries, ni, opt, bp);
+    host_program(bp, psi, lam, g);
+    for (size_t p = 0; p < n_params; p++) grad_out[p] = 0.0;
+    for (uint32_t i = 0; i < bp.n_grad; i++) grad_out[bp.grad_param[i]] += 2.0 * g[i];
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates, int fusion, int tile_qubits,
+                            int low_qubits, int64_t sweep_budget, uint64_t* n_passes, char* buf, size_t buflen)
+{
+    if (n < 1 || n > kMaxQubits) return fail(SVMI_ERR_ARG, "bad n");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    SVMI_TRY(check_gates(gates, n_gates, n));
+    Program prog;
+    plan_forward(gates, n_gates, std::max(n, kRegBits), debug_options(fusion, tile_qubits, low_qubits, sweep_budget),
+                 prog);
+    if (n_passes) *n_passes = prog.steps.size();
+    if (buf && buflen) {
+        const std::string s = describe_program(prog);
+        const size_t m = std::min(buflen - 1, s.size());
+        std::memcpy(buf, s.data(), m);
+        buf[m] = 0;
+    }
+    return SVMI_OK;
+}
+
+}  // extern "C"

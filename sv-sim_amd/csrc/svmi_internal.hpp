@@ -1,0 +1,250 @@
+// svmi_internal.hpp -- types shared by the host planner and the HIP kernels.
+//
+// Layout of the state in HBM: array-of-structures, one 16-byte c64 {re, im}
+// per amplitude, index = basis state (qubit q = bit q).  A 2^n state is
+// 16 * 2^n bytes (16 GiB at n = 30, 128 GiB at n = 33).
+//
+// A circuit is lowered (svmi_lower.cpp) to a list of "lowered ops" (dense,
+// diagonal or SRN on <= 5 qubits) and scheduled (svmi_plan.cpp) into sweeps.
+// A sweep is one read+write pass over the state: the state is cut into tiles
+// of 2^k amplitudes that share all bits outside the sweep's tile qubit set T;
+// every tile is staged in LDS and a sequence of phases is applied to it.  A
+// phase picks 4 tile bits as "register bits" R: each thread holds the 16
+// amplitudes of one group (fixed non-R bits) in registers, applies every op of
+// the phase, and writes the group back to LDS.  Non-diagonal ops of a phase
+// act on R only; diagonal ops may act on any qubit (tile bit or not).
+#pragma once
+
+#include <stdint.h>
+#include <stddef.h>
+
+#if defined(__HIPCC__)
+#define SVMI_HD __host__ __device__
+#else
+#define SVMI_HD
+#endif
+
+namespace svmi {
+
+struct alignas(16) c64 {
+    double re, im;
+};
+
+SVMI_HD inline c64 cmul(c64 a, c64 b) { return c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+SVMI_HD inline c64 cadd(c64 a, c64 b) { return c64{a.re + b.re, a.im + b.im}; }
+// acc + m * v
+SVMI_HD inline c64 cmac(c64 acc, c64 m, c64 v)
+{
+    acc.re += m.re * v.re;
+    acc.re -= m.im * v.im;
+    acc.im += m.re * v.im;
+    acc.im += m.im * v.re;
+    return acc;
+}
+// conj(a) * b
+SVMI_HD inline c64 cconjmul(c64 a, c64 b) { return c64{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re}; }
+
+constexpr int kRegBits = 4;                 // register bits per phase
+constexpr int kRegs = 1 << kRegBits;        // amplitudes per thread per group
+constexpr int kMaxTileQubits = 12;          // LDS tile: 2^12 * 16 B = 64 KiB
+constexpr int kMaxTileQubitsAdj = 11;       // two states (phi, lambda): 2 * 32 KiB
+constexpr int kThreads = 256;               // threads per tile workgroup
+constexpr int kMaxDenseQubits = 5;          // standalone dense ops
+constexpr int kMaxGradsPerSweep = 256;      // adjoint accumulators per sweep (LDS)
+
+// Device op types.
+enum DOpType : uint16_t {
+    DOP_NOP = 0,
+    DOP_DENSE = 1,   // dense 2^q x 2^q on register slots `smask`; coef = matrix
+    DOP_X = 2,       // bit flip on slot s0
+    DOP_CX = 3,      // control slot s0, target slot s1
+    DOP_SWAP = 4,    // slots s0, s1
+    DOP_SRN = 5,     // reference SRN quirk on slot s0 (real-linear only)
+    DOP_CPHASE = 6,  // amp *= coef[0] where all bits of the op are set
+    DOP_DIAG = 7,    // amp *= coef[idx], idx from the op's nq qubits
+    DOP_GRAD_DENSE = 8,  // adjoint: acc[grad] += Re sum conj(lam) (coef-matrix phi) on slots
+    DOP_GRAD_DIAG = 9,   // adjoint: acc[grad] += Re sum conj(lam) coef[idx] phi
+};
+
+// Which state(s) an op applies to in adjoint sweeps.
+enum : uint8_t { DOP_TO_PSI = 1, DOP_TO_LAM = 2, DOP_TO_BOTH = 3 };
+
+// One device op (uniform across the workgroup; read with scalar loads).
+struct DOp {
+    uint16_t type;
+    uint16_t smask;      // DENSE / GRAD_DENSE: register slot mask
+    uint8_t s0, s1;      // X/CX/SWAP/SRN slots
+    uint8_t nq;          // DIAG / CPHASE: number of qubits
+    uint8_t targets;     // adjoint: DOP_TO_* (forward sweeps: DOP_TO_PSI)
+    uint32_t coef;       // offset into the coefficient array (c64 units)
+    uint32_t grad;       // GRAD_*: accumulator slot within the sweep
+    uint32_t lmask;      // CPHASE: tile-local bits that must be 1
+    uint64_t emask;      // CPHASE/DIAG: physical bits outside the tile that must be 1 (CPHASE)
+    int8_t qloc[5];      // DIAG: tile-local bit of qubit i, or -1 if outside the tile
+    uint8_t qphys[5];    // DIAG: physical position of qubit i
+    uint8_t pad[6];
+};
+static_assert(sizeof(DOp) == 48, "DOp layout");
+
+struct PhaseInfo {
+    uint8_t r[4];        // register bits (tile-local positions, ascending)
+    uint32_t op_begin, op_end;
+};
+
+struct SweepInfo {
+    uint32_t phase_begin, phase_end;
+    uint32_t k;          // tile qubits
+    uint32_t c;          // low physical qubits 0..c-1 always in the tile
+    uint32_t n_ext;      // number of physical qubits outside the tile
+    uint32_t hitab;      // offset into the hi-offset table (2^(k-c) entries)
+    uint32_t n_grad;     // adjoint: accumulators used by this sweep
+    uint32_t grad_base;  // adjoint: first global accumulator of this sweep
+    uint8_t ext[40];     // physical positions outside the tile (ascending)
+    uint64_t n_tiles;
+};
+
+// A standalone op (run by its own kernel on the whole state, outside sweeps).
+struct StandaloneOp {
+    int kind;            // 0 dense, 1 diag, 2 srn
+    int q;               // qubits
+    int qubits[5];       // physical positions; local bit i <-> qubits[i]
+    uint32_t coef;
+};
+
+// ----------------------------------------------------------------------------
+// Op application on one register group (shared by kernels and host checker).
+// a[rho], rho = 4 register bits; bit s of rho <-> register slot s <-> tile bit r[s].
+// ----------------------------------------------------------------------------
+
+template <int S>
+struct SlotMask {
+    static constexpr int q = (S & 1) + ((S >> 1) & 1) + ((S >> 2) & 1) + ((S >> 3) & 1);
+    // i-th set bit position of S
+    static constexpr int bit(int i)
+    {
+        int c = 0;
+        for (int b = 0; b < 4; b++)
+            if ((S >> b) & 1) {
+                if (c == i) return b;
+                c++;
+            }
+        return -1;
+    }
+    // spread the low q bits of j onto the set bits of S
+    static constexpr int spread(int j)
+    {
+        int r = 0, c = 0;
+        for (int b = 0; b < 4; b++)
+            if ((S >> b) & 1) {
+                if ((j >> c) & 1) r |= 1 << b;
+                c++;
+            }
+        return r;
+    }
+    // spread the low 4-q bits of o onto the clear bits of S
+    static constexpr int spread_out(int o)
+    {
+        int r = 0, c = 0;
+        for (int b = 0; b < 4; b++)
+            if (!((S >> b) & 1)) {
+                if ((o >> c) & 1) r |= 1 << b;
+                c++;
+            }
+        return r;
+    }
+};
+
+// Dense 2^q x 2^q (row-major, matrix bit i <-> i-th set slot of S).
+template <int S>
+SVMI_HD inline void apply_dense(c64 (&a)[kRegs], const c64* __restrict__ M)
+{
+    constexpr int q = SlotMask<S>::q;
+    constexpr int D = 1 << q;
+#pragma unroll
+    for (int o = 0; o < (kRegs >> q); o++) {
+        c64 v[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) v[j] = a[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)];
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            c64 acc = c64{0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < D; j++) acc = cmac(acc, M[i * D + j], v[j]);
+            a[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i)] = acc;
+        }
+    }
+}
+
+// Re sum_groups conj(l) . (M p) over the slots of S (state untouched).
+template <int S>
+SVMI_HD inline double grad_dense(const c64 (&p)[kRegs], const c64 (&l)[kRegs], const c64* __restrict__ M)
+{
+    constexpr int q = SlotMask<S>::q;
+    constexpr int D = 1 << q;
+    double sum = 0.0;
+#pragma unroll
+    for (int o = 0; o < (kRegs >> q); o++) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            c64 acc = c64{0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < D; j++)
+                acc = cmac(acc, M[i * D + j], p[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)]);
+            const c64 lv = l[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i)];
+            sum += lv.re * acc.re + lv.im * acc.im;
+        }
+    }
+    return sum;
+}
+
+template <int s>
+SVMI_HD inline void apply_x(c64 (&a)[kRegs])
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (!((r >> s) & 1)) {
+            const c64 t = a[r];
+            a[r] = a[r | (1 << s)];
+            a[r | (1 << s)] = t;
+        }
+}
+
+template <int c, int t>
+SVMI_HD inline void apply_cx(c64 (&a)[kRegs])
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (((r >> c) & 1) && !((r >> t) & 1)) {
+            const c64 tmp = a[r];
+            a[r] = a[r | (1 << t)];
+            a[r | (1 << t)] = tmp;
+        }
+}
+
+template <int s0, int s1>
+SVMI_HD inline void apply_swap(c64 (&a)[kRegs])
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (((r >> s0) & 1) && !((r >> s1) & 1)) {
+            const int r2 = r ^ (1 << s0) ^ (1 << s1);
+            const c64 tmp = a[r];
+            a[r] = a[r2];
+            a[r2] = tmp;
+        }
+}
+
+// Reference SRN_GATE (svsim_cpu_omp.hpp:866-879), as coded there.
+template <int s>
+SVMI_HD inline void apply_srn(c64 (&a)[kRegs])
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (!((r >> s) & 1)) {
+            const c64 a0 = a[r], a1 = a[r | (1 << s)];
+            a[r] = c64{0.5 * (a0.re + a1.re), 0.5 * (a0.im - a1.im)};
+            a[r | (1 << s)] = c64{0.5 * (a0.re + a1.re), 0.5 * (-a0.im + a1.im)};
+        }
+}
+
+}  // namespace svmi

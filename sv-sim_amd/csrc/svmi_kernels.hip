@@ -1,0 +1,470 @@
+// svmi_kernels.hip -- CDNA4 (gfx950) kernels of the state-vector engine.
+//
+// k_sweep<NS>: one tiled sweep (the hot kernel).  One workgroup per tile of
+//   2^k amplitudes, 2^(k-4) threads; the tile is staged in LDS (swizzled),
+//   each phase gathers one 16-amplitude register group per thread, runs the
+//   phase's ops in registers and writes the group back; after the last phase
+//   the tile goes back to HBM.  Loads/stores are 16-byte per lane and the
+//   low c physical qubits of every tile are contiguous, so each group of
+//   2^c lanes reads one 2^c*16-byte segment.  NS = 2 runs the adjoint sweep
+//   over (phi, lambda) and accumulates gradient partial sums per tile.
+// Standalone ops (5-qubit dense, per-op mode), sampling, Pauli sums and
+//   state fills are simple grid-stride kernels.
+#include <hip/hip_runtime.h>
+
+#include "svmi_exec.hpp"
+#include "svmi_kernels.hpp"
+
+namespace svmi {
+
+// ------------------------------------------------------------------ sweeps
+
+template <int NS>
+__global__ void __launch_bounds__(kThreads)
+k_sweep(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw,
+        const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
+        const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
+        double* __restrict__ grad_partials, uint64_t tile_offset)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t k = sw.k;
+    const uint32_t tsize = 1u << k;
+    c64* t0 = reinterpret_cast<c64*>(smem);
+    c64* t1 = t0 + (NS == 2 ? tsize : 0);
+    double* gacc = reinterpret_cast<double*>(t0 + tsize * NS);
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nt = blockDim.x;            // == 2^(k-4)
+    const uint32_t nwaves = (nt + 63) / 64;
+    const uint64_t tile = tile_offset + blockIdx.x;
+    const uint64_t base = tile_base(sw, tile);
+    const uint32_t cmask = (1u << sw.c) - 1u;
+    const uint64_t* ht = hitab + sw.hitab;
+
+#pragma unroll
+    for (int j = 0; j < kRegs; j++) {
+        const uint32_t l = tid + nt * j;
+        const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
+        t0[lds_slot(l)] = s0[g];
+        if (NS == 2) t1[lds_slot(l)] = s1[g];
+    }
+    if (NS == 2)
+        for (uint32_t i = tid; i < sw.n_grad * nwaves; i += nt) gacc[i] = 0.0;
+    __syncthreads();
+
+    for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
+        const PhaseInfo ph = phases[p];
+        uint32_t off[kRegs];
+        phase_offsets(ph, off);
+        const uint32_t goff = group_offset(ph, tid);
+        c64 a[kRegs];
+        c64 b[kRegs];
+#pragma unroll
+        for (int r = 0; r < kRegs; r++) {
+            a[r] = t0[lds_slot(goff | off[r])];
+            if (NS == 2) b[r] = t1[lds_slot(goff | off[r])];
+        }
+        for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
+            const DOp op = ops[o];
+            if (NS == 2 && (op.type == DOP_GRAD_DENSE || op.type == DOP_GRAD_DIAG)) {
+                double v = exec_grad_op(op, a, b, coef, goff, off, base);
+                const int width = nt < 64 ? int(nt) : 64;
+                for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
+                if ((tid & 63u) == 0) gacc[op.grad * nwaves + (tid >> 6)] += v;
+            } else {
+                exec_state_op(op, a, coef, goff, off, base);
+                if (NS == 2) exec_state_op(op, b, coef, goff, off, base);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRegs; r++) {
+            t0[lds_slot(goff | off[r])] = a[r];
+            if (NS == 2) t1[lds_slot(goff | off[r])] = b[r];
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int j = 0; j < kRegs; j++) {
+        const uint32_t l = tid + nt * j;
+        const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
+        s0[g] = t0[lds_slot(l)];
+        if (NS == 2) s1[g] = t1[lds_slot(l)];
+    }
+    if (NS == 2) {
+        for (uint32_t i = tid; i < sw.n_grad; i += nt) {
+            double v = 0.0;
+            for (uint32_t w = 0; w < nwaves; w++) v += gacc[i * nwaves + w];
+            grad_partials[tile * sw.n_grad + i] = v;
+        }
+    }
+}
+
+static size_t g_max_smem = 64 * 1024;  // dynamic LDS the sweep kernels may use
+
+size_t sweep_smem_bytes(const SweepInfo& sw, int ns)
+{
+    const size_t nt = size_t(1) << (sw.k - kRegBits);
+    const size_t nwaves = (nt + 63) / 64;
+    return (size_t(16) << sw.k) * ns + size_t(8) * sw.n_grad * nwaves;
+}
+
+hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInfo& sw,
+                        const PhaseInfo* phases, const DOp* ops, const c64* coef,
+                        const uint64_t* hitab, double* grad_partials)
+{
+    const uint32_t nt = 1u << (sw.k - kRegBits);
+    const size_t smem = sweep_smem_bytes(sw, ns);
+    if (smem > g_max_smem) return hipErrorInvalidConfiguration;
+    const uint64_t max_grid = 1ull << 30;
+    for (uint64_t t0 = 0; t0 < sw.n_tiles; t0 += max_grid) {
+        const uint64_t cnt = sw.n_tiles - t0 < max_grid ? sw.n_tiles - t0 : max_grid;
+        if (ns == 1)
+            hipLaunchKernelGGL(k_sweep<1>, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases,
+                               ops, coef, hitab, grad_partials, t0);
+        else
+            hipLaunchKernelGGL(k_sweep<2>, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases,
+                               ops, coef, hitab, grad_partials, t0);
+    }
+    return hipGetLastError();
+}
+
+size_t sweep_smem_limit() { return g_max_smem; }
+
+// Raise the dynamic-LDS limit of the sweep kernels to what the device allows
+// per workgroup (gfx950: 160 KiB per CU); 64 KiB always works.
+hipError_t init_sweep_kernels()
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0)
+        optin = 64 * 1024;
+    size_t want = size_t(optin);
+    if (want > 160 * 1024) want = 160 * 1024;
+    if (want > 64 * 1024) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep<1>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep<2>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            want = 64 * 1024;
+        }
+    }
+    g_max_smem = want;
+    return hipSuccess;
+}
+
+// -------------------------------------------------------- standalone ops
+
+struct QArgs {
+    int q;
+    int qubits[5];      // op order
+    int sorted[5];      // ascending
+};
+
+template <int K>
+__global__ void __launch_bounds__(256)
+k_dense_full(c64* __restrict__ s, uint64_t ngroups, QArgs qa, const c64* __restrict__ M)
+{
+    constexpr int D = 1 << K;
+    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < ngroups;
+         g += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t b = g;
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const int p = qa.sorted[i];
+            b = ((b >> p) << (p + 1)) | (b & ((1ull << p) - 1ull));
+        }
+        uint64_t idx[D];
+        c64 v[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            uint64_t o = b;
+#pragma unroll
+            for (int i = 0; i < K; i++)
+                if ((j >> i) & 1) o |= 1ull << qa.qubits[i];
+            idx[j] = o;
+            v[j] = s[o];
+        }
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            c64 acc{0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < D; j++) acc = cmac(acc, M[r * D + j], v[j]);
+            s[idx[r]] = acc;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_diag_full(c64* __restrict__ s, uint64_t dim, QArgs qa, const c64* __restrict__ tab)
+{
+    for (uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; x < dim;
+         x += uint64_t(gridDim.x) * blockDim.x) {
+        uint32_t idx = 0;
+        for (int i = 0; i < qa.q; i++) idx |= uint32_t((x >> qa.qubits[i]) & 1u) << i;
+        s[x] = cmul(s[x], tab[idx]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_srn_full(c64* __restrict__ s, uint64_t half, int q)
+{
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < half;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t p0 = ((i >> q) << (q + 1)) | (i & ((1ull << q) - 1ull));
+        const uint64_t p1 = p0 | (1ull << q);
+        const c64 a0 = s[p0], a1 = s[p1];
+        s[p0] = c64{0.5 * (a0.re + a1.re), 0.5 * (a0.im - a1.im)};
+        s[p1] = c64{0.5 * (a0.re + a1.re), 0.5 * (-a0.im + a1.im)};
+    }
+}
+
+static unsigned grid_for(uint64_t work, unsigned block)
+{
+    const uint64_t g = (work + block - 1) / block;
+    const uint64_t cap = 65536;
+    return unsigned(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+hipError_t launch_standalone(hipStream_t st, c64* s, int n, const StandaloneOp& so, const c64* coef)
+{
+    QArgs qa;
+    qa.q = so.q;
+    for (int i = 0; i < 5; i++) qa.qubits[i] = i < so.q ? so.qubits[i] : 0;
+    for (int i = 0; i < 5; i++) qa.sorted[i] = qa.qubits[i];
+    for (int i = 0; i < so.q; i++)
+        for (int j = i + 1; j < so.q; j++)
+            if (qa.sorted[j] < qa.sorted[i]) {
+                const int t = qa.sorted[i];
+                qa.sorted[i] = qa.sorted[j];
+                qa.sorted[j] = t;
+            }
+    const uint64_t dim = 1ull << n;
+    if (so.kind == 2) {
+        hipLaunchKernelGGL(k_srn_full, dim3(grid_for(dim / 2, 256)), dim3(256), 0, st, s, dim / 2, so.qubits[0]);
+    } else if (so.kind == 1) {
+        hipLaunchKernelGGL(k_diag_full, dim3(grid_for(dim, 256)), dim3(256), 0, st, s, dim, qa, coef + so.coef);
+    } else {
+        const uint64_t ng = dim >> so.q;
+        const dim3 gr(grid_for(ng, 256)), bl(256);
+        const c64* M = coef + so.coef;
+        switch (so.q) {
+        case 1: hipLaunchKernelGGL(k_dense_full<1>, gr, bl, 0, st, s, ng, qa, M); break;
+        case 2: hipLaunchKernelGGL(k_dense_full<2>, gr, bl, 0, st, s, ng, qa, M); break;
+        case 3: hipLaunchKernelGGL(k_dense_full<3>, gr, bl, 0, st, s, ng, qa, M); break;
+        case 4: hipLaunchKernelGGL(k_dense_full<4>, gr, bl, 0, st, s, ng, qa, M); break;
+        default: hipLaunchKernelGGL(k_dense_full<5>, gr, bl, 0, st, s, ng, qa, M); break;
+        }
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ state utils
+
+__global__ void __launch_bounds__(256) k_fill_basis(c64* __restrict__ s, uint64_t dim)
+{
+    for (uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; x < dim;
+         x += uint64_t(gridDim.x) * blockDim.x)
+        s[x] = c64{x == 0 ? 1.0 : 0.0, 0.0};
+}
+
+__global__ void __launch_bounds__(256) k_copy(c64* __restrict__ d, const c64* __restrict__ s, uint64_t dim)
+{
+    for (uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; x < dim;
+         x += uint64_t(gridDim.x) * blockDim.x)
+        d[x] = s[x];
+}
+
+hipError_t launch_fill_basis(hipStream_t st, c64* s, uint64_t dim)
+{
+    hipLaunchKernelGGL(k_fill_basis, dim3(grid_for(dim, 256)), dim3(256), 0, st, s, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(hipStream_t st, c64* d, const c64* s, uint64_t dim)
+{
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(dim, 256)), dim3(256), 0, st, d, s, dim);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------------- sampling
+//
+// Scan definition (identical in both kernels, fixed operation order):
+//   p_x = (re*re) + (im*im) with unfused IEEE ops (the reference's
+//   expression, svsim_cpu_omp.hpp:345);
+//   block of B = nthr*chunk amplitudes; thread t owns a contiguous chunk;
+//   T_t = sequential sum of its chunk; P_t = exclusive prefix of T over the
+//   threads (wave64 shuffle scan, then waves in order);
+//   L_j = P_t + p_(first) + ... + p_j accumulated sequentially;
+//   block total = L of the block's last amplitude.
+// Global scan S_j = O_b + L_j with O_b the host's sequential prefix of the
+// block totals; a draw r selects the first j with S_j > r (0 if none), which
+// is the reference's interval rule scan[j] <= r < scan[j+1].
+
+__device__ inline double prob_rn(c64 a) { return __dadd_rn(__dmul_rn(a.re, a.re), __dmul_rn(a.im, a.im)); }
+
+__device__ double thread_prefix(const c64* __restrict__ blk, uint32_t chunk, double* lds_w)
+{
+    const uint32_t t = threadIdx.x, nthr = blockDim.x;
+    double T = 0.0;
+    for (uint32_t i = 0; i < chunk; i++) T = __dadd_rn(T, prob_rn(blk[size_t(t) * chunk + i]));
+    const int width = nthr < 64 ? int(nthr) : 64;
+    const int lane = int(t & 63u);
+    double inc = T;
+    for (int d = 1; d < width; d <<= 1) {
+        const double u = __shfl_up(inc, d, width);
+        if (lane >= d) inc = __dadd_rn(inc, u);
+    }
+    double exc = __shfl_up(inc, 1, width);
+    if (lane == 0) exc = 0.0;
+    const uint32_t w = t >> 6, nw = (nthr + 63) / 64;
+    if (lane == width - 1) lds_w[w] = inc;
+    __syncthreads();
+    double wpre = 0.0;
+    for (uint32_t v = 0; v < w; v++) wpre = __dadd_rn(wpre, lds_w[v]);
+    (void)nw;
+    __syncthreads();
+    return w == 0 ? exc : __dadd_rn(wpre, exc);
+}
+
+__global__ void __launch_bounds__(256)
+k_block_totals(const c64* __restrict__ s, uint32_t chunk, double* __restrict__ totals)
+{
+    __shared__ double lds_w[4];
+    const uint64_t b = blockIdx.x;
+    const c64* blk = s + b * uint64_t(blockDim.x) * chunk;
+    double acc = thread_prefix(blk, chunk, lds_w);
+    for (uint32_t i = 0; i < chunk; i++) acc = __dadd_rn(acc, prob_rn(blk[size_t(threadIdx.x) * chunk + i]));
+    if (threadIdx.x == blockDim.x - 1) totals[b] = acc;
+}
+
+// One workgroup per (block, shot range).  job = {block, first shot, n shots}.
+__global__ void __launch_bounds__(256)
+k_sample(const c64* __restrict__ s, uint32_t chunk, const uint64_t* __restrict__ jobs,
+         const double* __restrict__ offsets, const double* __restrict__ draws,
+         uint64_t* __restrict__ out)
+{
+    __shared__ double lds_w[4];
+    __shared__ unsigned long long best;
+    const uint64_t b = jobs[3 * blockIdx.x + 0];
+    const uint64_t first = jobs[3 * blockIdx.x + 1];
+    const uint64_t cnt = jobs[3 * blockIdx.x + 2];
+    const uint64_t bsize = uint64_t(blockDim.x) * chunk;
+    const c64* blk = s + b * bsize;
+    const double P = thread_prefix(blk, chunk, lds_w);
+    const double O = offsets[blockIdx.x];
+    for (uint64_t q = 0; q < cnt; q++) {
+        const double r = draws[first + q];
+        if (threadIdx.x == 0) best = ~0ull;
+        __syncthreads();
+        double acc = P;
+        uint64_t found = ~0ull;
+        for (uint32_t i = 0; i < chunk; i++) {
+            acc = __dadd_rn(acc, prob_rn(blk[size_t(threadIdx.x) * chunk + i]));
+            if (__dadd_rn(O, acc) > r) {
+                found = uint64_t(threadIdx.x) * chunk + i;
+                break;
+            }
+        }
+        if (found != ~0ull) atomicMin(&best, (unsigned long long)found);
+        __syncthreads();
+        if (threadIdx.x == 0) out[first + q] = best == ~0ull ? 0 : b * bsize + best;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_block_totals(hipStream_t st, const c64* s, uint64_t nblocks, uint32_t nthr,
+                               uint32_t chunk, double* totals)
+{
+    for (uint64_t b0 = 0; b0 < nblocks; b0 += (1ull << 30)) {
+        const uint64_t cnt = nblocks - b0 < (1ull << 30) ? nblocks - b0 : (1ull << 30);
+        hipLaunchKernelGGL(k_block_totals, dim3(uint32_t(cnt)), dim3(nthr), 0, st,
+                           s + b0 * uint64_t(nthr) * chunk, chunk, totals + b0);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sample(hipStream_t st, const c64* s, uint32_t nthr, uint32_t chunk,
+                         const uint64_t* jobs, uint32_t njobs, const double* offsets,
+                         const double* draws, uint64_t* out)
+{
+    if (njobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sample, dim3(njobs), dim3(nthr), 0, st, s, chunk, jobs, offsets, draws, out);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------- Pauli sums
+//
+// lam[y] = sum_g sum_{t in g} c'_t (-1)^{|x & z_t|} psi[x], x = y ^ xmask_g,
+// c'_t = coeff_t * i^{|xmask & zmask_t|}.  Block partials of
+// Re(conj(psi[y]) lam[y]) give <psi|H|psi>.
+
+__global__ void __launch_bounds__(256)
+k_pauli(const c64* __restrict__ psi, c64* __restrict__ lam, uint64_t dim,
+        const PauliGroup* __restrict__ groups, uint32_t ngroups, const PauliTerm* __restrict__ terms,
+        double* __restrict__ partials)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (uint64_t y = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; y < dim;
+         y += uint64_t(gridDim.x) * blockDim.x) {
+        c64 l{0.0, 0.0};
+        for (uint32_t g = 0; g < ngroups; g++) {
+            const PauliGroup G = groups[g];
+            const uint64_t x = y ^ G.xmask;
+            double fr = 0.0, fi = 0.0;
+            for (uint32_t t = G.begin; t < G.end; t++) {
+                const PauliTerm T = terms[t];
+                const bool neg = __popcll(x & T.zmask) & 1;
+                fr += neg ? -T.cre : T.cre;
+                fi += neg ? -T.cim : T.cim;
+            }
+            l = cmac(l, c64{fr, fi}, psi[x]);
+        }
+        if (lam) lam[y] = l;
+        const c64 p = psi[y];
+        acc += p.re * l.re + p.im * l.im;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+uint32_t pauli_grid(uint64_t dim) { return grid_for(dim, 256) > 2048 ? 2048 : grid_for(dim, 256); }
+
+hipError_t launch_pauli(hipStream_t st, const c64* psi, c64* lam, uint64_t dim, const PauliGroup* groups,
+                        uint32_t ngroups, const PauliTerm* terms, double* partials)
+{
+    hipLaunchKernelGGL(k_pauli, dim3(pauli_grid(dim)), dim3(256), 0, st, psi, lam, dim, groups, ngroups,
+                       terms, partials);
+    return hipGetLastError();
+}
+
+// Sum of per-tile gradient partials, one thread per accumulator (fixed order).
+__global__ void k_reduce_grads(const double* __restrict__ partials, uint64_t ntiles, uint32_t ngrad,
+                               double* __restrict__ out)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrad) return;
+    double s = 0.0;
+    for (uint64_t t = 0; t < ntiles; t++) s += partials[t * ngrad + g];
+    out[g] = s;
+}
+
+hipError_t launch_reduce_grads(hipStream_t st, const double* partials, uint64_t ntiles, uint32_t ngrad,
+                               double* out)
+{
+    if (ngrad == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_grads, dim3((ngrad + 63) / 64), dim3(64), 0, st, partials, ntiles, ngrad, out);
+    return hipGetLastError();
+}
+
+}  // namespace svmi

@@ -1,0 +1,64 @@
+// svmi_plan.hpp -- scheduling of lowered ops into device sweeps.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "svmi_internal.hpp"
+#include "svmi_lower.hpp"
+
+namespace svmi {
+
+struct PlanOptions {
+    bool fusion = true;        // tiled multi-op sweeps; false = one pass per op
+    int tile_qubits = 12;      // k: qubits per LDS tile
+    int low_qubits = 4;        // c: physical qubits 0..c-1 in every tile
+    long sweep_budget = 0;     // per-amplitude FMA budget of one sweep (0 = none)
+};
+
+// One step of a program: a tiled sweep or a standalone full-state op.
+struct Step {
+    enum Kind { SWEEP = 0, STANDALONE = 1 } kind;
+    uint32_t index;
+};
+
+// Host copy of a device program.
+struct Program {
+    int n = 0;
+    int n_states = 1;                  // 2 for adjoint sweeps (phi, lambda)
+    std::vector<Step> steps;
+    std::vector<SweepInfo> sweeps;
+    std::vector<PhaseInfo> phases;
+    std::vector<DOp> ops;
+    std::vector<c64> coef;
+    std::vector<uint64_t> hitab;
+    std::vector<StandaloneOp> standalone;
+    // adjoint: accumulator g of sweep s -> gradient parameter index
+    std::vector<int32_t> grad_param;
+    uint32_t n_grad = 0;
+    uint64_t n_lowered = 0;            // ops after lowering + peephole fusion
+};
+
+// Merge runs of single-qubit ops on one qubit (no other op on that qubit in
+// between) into one op.  Ops on other qubits commute past them.
+void fuse_single_qubit_runs(const std::vector<LOp>& in, std::vector<LOp>& out);
+
+// Forward program for n qubits.
+void build_forward_program(const std::vector<LOp>& ops, int n, const PlanOptions& opt, Program& prog);
+
+// An entry of an adjoint (backward) schedule: the inverse of one gate applied
+// to phi and lambda, or a gradient accumulation <lam| dU |phi> between them.
+struct AdjEntry {
+    LOp op;             // U^dagger (targets phi and lambda) or dU (grad)
+    bool grad = false;
+    int param = -1;     // gradient parameter (grad entries)
+};
+
+// Backward program over (phi, lambda); entries in execution order.
+void build_adjoint_program(const std::vector<AdjEntry>& entries, int n, const PlanOptions& opt,
+                           Program& prog);
+
+// Human-readable summary (sweeps, phases, ops per sweep).
+std::string describe_program(const Program& prog);
+
+}  // namespace svmi

@@ -1,0 +1,289 @@
+"""Python access to the MI355X state-vector engine through its C ABI.
+
+``svmi.Engine`` wraps one ``svmi_handle`` of ``libsvmi.so`` (include/svmi.h):
+device state, uploaded circuit, run, measure with fed draws, Pauli
+expectations and adjoint gradients.  Gate records use the reference layout
+(svsim/src/svsim_cpu_omp.hpp:103-194).  The library must be built
+(``python sv-sim_amd/build.py``); nothing here falls back to the CPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libsvmi.so")
+
+# Reference enum OP (svsim_cpu_omp.hpp:46-52).
+OP_NAMES = ["U3", "U2", "U1", "CX", "ID", "X", "Y", "Z", "H", "S",
+            "SDG", "T", "TDG", "RX", "RY", "RZ", "CZ", "CY", "SWAP", "CH",
+            "CCX", "CSWAP", "CRX", "CRY", "CRZ", "CU1", "CU3", "RXX", "RZZ", "RCCX",
+            "RC3X", "C3X", "C3SQRTX", "C4X", "R", "SRN", "W", "RYY"]
+OP = {name: i for i, name in enumerate(OP_NAMES)}
+ARITY = {name: 1 for name in OP_NAMES}
+ARITY.update({n: 2 for n in ("CX", "CZ", "CY", "SWAP", "CH", "CRX", "CRY", "CRZ", "CU1", "CU3",
+                             "RXX", "RZZ", "RYY")})
+ARITY.update({"CCX": 3, "CSWAP": 3, "RCCX": 3, "RC3X": 4, "C3X": 4, "C3SQRTX": 4, "C4X": 5})
+
+GATE_DTYPE = np.dtype([("op", "<i4"), ("qb", "<i4", (5,)),
+                       ("theta", "<f8"), ("phi", "<f8"), ("lambda", "<f8")])
+PAULI_DTYPE = np.dtype([("coeff", "<f8"), ("xmask", "<u8"), ("zmask", "<u8")])
+
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
+          5: "ERR_DEVICE", 6: "ERR_OOM", 7: "ERR_UNSUPPORTED"}
+
+OPT_FUSION, OPT_TILE_QUBITS, OPT_LOW_QUBITS, OPT_SWEEP_FLOPS, OPT_PROFILE = 0, 1, 2, 3, 4
+
+# Every exported entry point (include/svmi.h, include/svmi_debug.h).
+EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
+           "svmi_set_state", "svmi_get_state", "svmi_upload", "svmi_clear_circuit",
+           "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
+           "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version",
+           "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_plan"]
+
+
+class SvmiError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("sweeps", ctypes.c_uint64), ("kernel_launches", ctypes.c_uint64),
+                ("ops", ctypes.c_uint64), ("gates", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+                ("sweep_launches", ctypes.c_uint64), ("sweep_ms", ctypes.c_double),
+                ("other_launches", ctypes.c_uint64), ("other_ms", ctypes.c_double)]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("n_qubits", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("gates", ctypes.c_void_p), ("n_gates", ctypes.c_size_t),
+                ("param_slots", ctypes.c_void_p), ("n_params", ctypes.c_size_t),
+                ("terms", ctypes.c_void_p), ("n_terms", ctypes.c_size_t)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+
+
+def lib():
+    """Load libsvmi.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run python sv-sim_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        st = ctypes.c_int
+        L.svmi_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.svmi_destroy.argtypes = [_vp]
+        L.svmi_num_qubits.argtypes = [_vp]
+        L.svmi_num_qubits.restype = ctypes.c_int
+        L.svmi_reset_state.argtypes = [_vp]
+        L.svmi_set_state.argtypes = [_vp, _vp, _vp]
+        L.svmi_get_state.argtypes = [_vp, _vp, _vp]
+        L.svmi_upload.argtypes = [_vp, _vp, _sz]
+        L.svmi_clear_circuit.argtypes = [_vp]
+        L.svmi_run.argtypes = [_vp]
+        L.svmi_measure.argtypes = [_vp, _vp, _sz, _vp, _vp]
+        L.svmi_expval.argtypes = [_vp, _vp, _sz, _vp]
+        L.svmi_adjoint.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _vp]
+        L.svmi_batch_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _vp]
+        L.svmi_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int64]
+        L.svmi_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
+        L.svmi_last_error.restype = ctypes.c_char_p
+        L.svmi_debug_emulate.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int64, _vp, _vp]
+        L.svmi_debug_emulate_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, _vp, _sz, ctypes.c_int,
+                                                 ctypes.c_int, _vp, _vp, _vp, _vp]
+        L.svmi_debug_plan.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name not in ("svmi_num_qubits", "svmi_last_error", "svmi_abi_version"):
+                f.restype = st
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise SvmiError(status, lib().svmi_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None and a.size else None
+
+
+def as_gates(gates):
+    """GATE_DTYPE array from a GATE_DTYPE array or (op, qubits, theta, phi, lambda) tuples."""
+    if isinstance(gates, np.ndarray) and gates.dtype == GATE_DTYPE:
+        return np.ascontiguousarray(gates)
+    arr = np.zeros(len(gates), dtype=GATE_DTYPE)
+    for i, g in enumerate(gates):
+        op = OP[g[0]] if isinstance(g[0], str) else int(g[0])
+        qs = list(g[1])
+        arr[i]["op"] = op
+        arr[i]["qb"][: len(qs)] = qs
+        vals = (list(g[2:5]) + [0.0, 0.0, 0.0])[:3]
+        arr[i]["theta"], arr[i]["phi"], arr[i]["lambda"] = vals
+    return arr
+
+
+def as_terms(terms):
+    if isinstance(terms, np.ndarray) and terms.dtype == PAULI_DTYPE:
+        return np.ascontiguousarray(terms)
+    arr = np.zeros(len(terms), dtype=PAULI_DTYPE)
+    for i, (c, x, z) in enumerate(terms):
+        arr[i] = (c, x, z)
+    return arr
+
+
+class Engine:
+    """One device state vector of n qubits on HIP device `device`."""
+
+    def __init__(self, n_qubits, device=0):
+        h = _vp()
+        _check(lib().svmi_create(int(n_qubits), int(device), ctypes.byref(h)))
+        self._h = h
+        self.n = int(n_qubits)
+        self.n_gates = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().svmi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def reset_state(self):
+        _check(lib().svmi_reset_state(self._h))
+
+    def set_state(self, re, im):
+        re = np.ascontiguousarray(re, dtype=np.float64)
+        im = np.ascontiguousarray(im, dtype=np.float64)
+        if re.size != (1 << self.n) or im.size != (1 << self.n):
+            raise ValueError("state must have 2^n entries")
+        _check(lib().svmi_set_state(self._h, _ptr(re), _ptr(im)))
+
+    def get_state(self):
+        re = np.empty(1 << self.n)
+        im = np.empty(1 << self.n)
+        _check(lib().svmi_get_state(self._h, _ptr(re), _ptr(im)))
+        return re, im
+
+    def upload(self, gates):
+        g = as_gates(gates)
+        _check(lib().svmi_upload(self._h, _ptr(g), len(g)))
+        self.n_gates = len(g)
+
+    def clear_circuit(self):
+        _check(lib().svmi_clear_circuit(self._h))
+
+    def run(self):
+        _check(lib().svmi_run(self._h))
+
+    def measure(self, uniforms):
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        out = np.zeros(len(u), dtype=np.uint64)
+        tot = ctypes.c_double(0.0)
+        _check(lib().svmi_measure(self._h, _ptr(u), len(u), _ptr(out), ctypes.byref(tot)))
+        return out, tot.value
+
+    def expval(self, terms):
+        t = as_terms(terms)
+        out = ctypes.c_double(0.0)
+        _check(lib().svmi_expval(self._h, _ptr(t), len(t), ctypes.byref(out)))
+        return out.value
+
+    def adjoint(self, terms, param_slots, n_params):
+        t = as_terms(terms)
+        ps = np.ascontiguousarray(param_slots, dtype=np.int32).reshape(-1)
+        if ps.size != 3 * self.n_gates:
+            raise ValueError("param_slots needs 3 entries per uploaded gate")
+        grad = np.zeros(n_params)
+        e = ctypes.c_double(0.0)
+        _check(lib().svmi_adjoint(self._h, _ptr(t), len(t), _ptr(ps), int(n_params), ctypes.byref(e),
+                                  _ptr(grad) if n_params else None))
+        return e.value, grad
+
+    def set_option(self, option, value):
+        _check(lib().svmi_set_option(self._h, int(option), int(value)))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().svmi_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+
+def batch_adjoint(items, device=0):
+    """items: list of dicts {n, gates, param_slots, n_params, terms}.
+    Returns (expvals, list of gradient arrays)."""
+    keep = []
+    arr = (BatchItem * len(items))()
+    total = 0
+    for i, it in enumerate(items):
+        g = as_gates(it["gates"])
+        ps = np.ascontiguousarray(it["param_slots"], dtype=np.int32).reshape(-1)
+        t = as_terms(it["terms"])
+        keep += [g, ps, t]
+        arr[i].n_qubits = int(it["n"])
+        arr[i].gates = g.ctypes.data
+        arr[i].n_gates = len(g)
+        arr[i].param_slots = ps.ctypes.data if ps.size else None
+        arr[i].n_params = int(it["n_params"])
+        arr[i].terms = t.ctypes.data if t.size else None
+        arr[i].n_terms = len(t)
+        total += int(it["n_params"])
+    ev = np.zeros(len(items))
+    gr = np.zeros(max(total, 1))
+    _check(lib().svmi_batch_adjoint(int(device), ctypes.cast(arr, _vp), len(items), _ptr(ev), _ptr(gr)))
+    out, off = [], 0
+    for it in items:
+        out.append(gr[off:off + int(it["n_params"])].copy())
+        off += int(it["n_params"])
+    return ev, out
+
+
+# ---- host checks of the planner (no device; see include/svmi_debug.h) ----
+
+def emulate(n, gates, state, fusion=True, tile_qubits=12, low_qubits=4, sweep_budget=0):
+    g = as_gates(gates)
+    re = np.array(state[0], dtype=np.float64, copy=True)
+    im = np.array(state[1], dtype=np.float64, copy=True)
+    _check(lib().svmi_debug_emulate(int(n), _ptr(g), len(g), int(fusion), int(tile_qubits), int(low_qubits),
+                                    int(sweep_budget), _ptr(re), _ptr(im)))
+    return re, im
+
+
+def emulate_adjoint(n, gates, terms, param_slots, n_params, state, tile_qubits=12, low_qubits=4):
+    g = as_gates(gates)
+    t = as_terms(terms)
+    ps = np.ascontiguousarray(param_slots, dtype=np.int32).reshape(-1)
+    re = np.ascontiguousarray(state[0], dtype=np.float64)
+    im = np.ascontiguousarray(state[1], dtype=np.float64)
+    grad = np.zeros(n_params)
+    e = ctypes.c_double(0.0)
+    _check(lib().svmi_debug_emulate_adjoint(int(n), _ptr(g), len(g), _ptr(t), len(t), _ptr(ps), int(n_params),
+                                            int(tile_qubits), int(low_qubits), _ptr(re), _ptr(im),
+                                            ctypes.byref(e), _ptr(grad) if n_params else None))
+    return e.value, grad
+
+
+def plan(n, gates, fusion=True, tile_qubits=12, low_qubits=4, sweep_budget=0):
+    g = as_gates(gates)
+    passes = ctypes.c_uint64(0)
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(lib().svmi_debug_plan(int(n), _ptr(g), len(g), int(fusion), int(tile_qubits), int(low_qubits),
+                                 int(sweep_budget), ctypes.byref(passes), buf, len(buf)))
+    return passes.value, buf.value.decode()

@@ -1,0 +1,122 @@
+"""Host logic of the engine, checked on the CPU.
+
+svmi.emulate runs the engine's lowering + scheduler and then executes the
+device program on the host with the same op code the HIP kernels use
+(sv-sim_amd/csrc/svmi_exec.hpp), so these tests pin the planner, the gate
+lowering and the adjoint recurrences against the oracle without a GPU.
+"""
+import numpy as np
+import pytest
+
+import svmi
+from oracle import oracle as O
+from svmi import workloads as W
+
+
+def _all_ops_circuit(n, reps, rng):
+    gates = []
+    for _ in range(reps):
+        for op in range(38):
+            k = svmi.ARITY[svmi.OP_NAMES[op]]
+            if k > n:
+                continue
+            gates.append((op, list(rng.choice(n, size=k, replace=False)), *(rng.random(3) * 4 * np.pi - 2 * np.pi)))
+    return gates
+
+
+def _err(a, b):
+    return max(np.abs(a[0] - b[0]).max(), np.abs(a[1] - b[1]).max())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 11, 13])
+@pytest.mark.parametrize("fusion,tile,low", [(0, 12, 4), (1, 12, 4), (1, 6, 2), (1, 4, 0), (1, 9, 3)])
+def test_emulated_program_matches_oracle(n, fusion, tile, low):
+    rng = np.random.default_rng(100 + n)
+    gates = _all_ops_circuit(n, 3, rng)
+    st = W.random_state(n, seed=n)
+    assert _err(svmi.emulate(n, gates, st, fusion=fusion, tile_qubits=tile, low_qubits=low), O.run(n, gates, st)) < 1e-12
+
+
+def test_sweep_budget_splits_but_keeps_result():
+    n = 12
+    g = W.u3_brickwall(n=n, layers=6)
+    st = W.random_state(n, seed=1)
+    p0, _ = svmi.plan(n, g)
+    p1, _ = svmi.plan(n, g, sweep_budget=64)
+    assert p1 > p0
+    assert _err(svmi.emulate(n, g, st, sweep_budget=64), O.run(n, g, st)) < 1e-12
+
+
+def test_pass_counts_of_the_measured_configs():
+    # one pass per op without fusion; tiled sweeps collapse the circuits
+    assert svmi.plan(12, W.random_layers())[0] == 1
+    assert svmi.plan(30, W.u3_brickwall(), fusion=False)[0] == 890
+    assert svmi.plan(30, W.u3_brickwall())[0] <= 30
+    assert svmi.plan(33, W.qft())[0] <= 10
+    g, _, _, _ = W.hea_ising()
+    assert svmi.plan(20, g)[0] <= 20
+
+
+def test_srn_is_an_ordering_barrier():
+    # the reference SRN is real-linear only: phases on other qubits must not
+    # move across it
+    n = 4
+    gates = [(svmi.OP["T"], [2]), (svmi.OP["SRN"], [0]), (svmi.OP["S"], [0]), (svmi.OP["RX"], [2], 0.3)]
+    st = W.random_state(n, seed=3)
+    assert _err(svmi.emulate(n, gates, st), O.run(n, gates, st)) < 1e-14
+
+
+@pytest.mark.parametrize("n,layers,tile", [(4, 2, 12), (7, 3, 5), (10, 2, 11), (13, 2, 8)])
+def test_adjoint_matches_parameter_shift(n, layers, tile):
+    rng = np.random.default_rng(n)
+    gates, slots, P = W.hea(n, layers, rng)
+    terms = W.ising_terms(n, rng)
+    st = W.random_state(n, seed=n + 1)
+    E, g = svmi.emulate_adjoint(n, gates, terms, slots, P, st, tile_qubits=tile)
+    assert abs(E - O.expval(n, *O.run(n, gates, st), terms)) < 1e-12
+    assert np.abs(g - O.param_shift_grad(n, gates, slots, P, terms, state=st)).max() < 1e-12
+
+
+def test_adjoint_all_parametric_gates_vs_fd():
+    n = 5
+    rng = np.random.default_rng(9)
+    gates, slots, p = [], [], 0
+    for name in ["U3", "U2", "U1", "RX", "RY", "RZ", "CRX", "CRY", "CRZ", "CU1", "CU3", "RXX", "RZZ", "RYY",
+                 "H", "CX", "CCX", "SWAP", "C4X"]:
+        k = svmi.ARITY[name]
+        gates.append((svmi.OP[name], list(rng.choice(n, size=k, replace=False)), *(rng.random(3) * 6 - 3)))
+        slots.append([p, p + 1, p + 2])
+        p += 3
+    slots = np.array(slots)
+    terms = [(0.7, 0b00011, 0b00010), (-1.3, 0, 0b10100), (0.4, 0b11000, 0b01000), (0.9, 0b00100, 0)]
+    st = W.random_state(n, seed=4)
+    E, g = svmi.emulate_adjoint(n, gates, terms, slots, p, st)
+    G = O.as_gates(gates)
+    fd = np.zeros(p)
+    names = ("theta", "phi", "lambda")
+    for gi in range(len(G)):
+        for s in range(3):
+            vals = []
+            for h in (1e-5, -1e-5, 2e-5, -2e-5):
+                G2 = G.copy()
+                G2[gi][names[s]] += h
+                vals.append(O.expval(n, *O.run(n, G2, st), terms))
+            fd[slots[gi, s]] = (8 * (vals[0] - vals[1]) - (vals[2] - vals[3])) / 12e-5
+    assert abs(E - O.expval(n, *O.run(n, G, st), terms)) < 1e-13
+    assert np.abs(g - fd).max() < 1e-9
+
+
+def test_adjoint_rejects_non_unitary():
+    n = 3
+    gates = [(svmi.OP["R"], [0], 0.5)]
+    with pytest.raises(svmi.SvmiError) as e:
+        svmi.emulate_adjoint(n, gates, [(1.0, 0, 1)], [[0, -1, -1]], 1, W.random_state(n, 1))
+    assert e.value.status == 7
+
+
+def test_invalid_gates_are_errors_not_aborts():
+    st = W.random_state(3, 1)
+    for bad, status in [([(svmi.OP["CX"], [1, 1])], 2), ([(svmi.OP["H"], [3])], 2), ([(99, [0])], 3)]:
+        with pytest.raises(svmi.SvmiError) as e:
+            svmi.emulate(3, bad, st)
+        assert e.value.status == status
