@@ -753,11 +753,12 @@ void host_program(const Program& prog, std::vector<c64>& s0, std::vector<c64>& s
                     }
                     for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                         const DOp& op = prog.ops[o];
-                        if (op.type == DOP_GRAD_DENSE || op.type == DOP_GRAD_DIAG) {
-                            gsum[sw.grad_base + op.grad] += exec_grad_op(op, a, b, prog.coef.data(), goff, off, base);
+                        if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG) {
+                            gsum[sw.grad_base + op.h.grad] +=
+                                exec_grad_op(op.h, &op, a, b, prog.coef.data(), goff, off, base);
                         } else {
-                            exec_state_op(op, a, prog.coef.data(), goff, off, base);
-                            if (ns == 2) exec_state_op(op, b, prog.coef.data(), goff, off, base);
+                            exec_state_op(op.h, &op, a, prog.coef.data(), goff, off, base);
+                            if (ns == 2) exec_state_op(op.h, &op, b, prog.coef.data(), goff, off, base);
                         }
                     }
                     for (int r = 0; r < kRegs; r++) {

@@ -7,10 +7,26 @@
 
 namespace svmi {
 
+// Distinct no-op markers at both ends of every switch case keep LLVM from
+// hoisting/sinking the cases into one access with a runtime register index,
+// which would push the whole register group to scratch memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SVMI_FENCE(N) asm volatile("; svmi case %0" ::"i"(N))
+#else
+#define SVMI_FENCE(N) ((void)0)
+#endif
+#define SVMI_CASE_ID(LABEL, ID, STMT) \
+    case LABEL:                         \
+        SVMI_FENCE(ID);                 \
+        STMT;                           \
+        SVMI_FENCE(ID);                 \
+        break;
+#define SVMI_CASE(N, STMT) SVMI_CASE_ID(N, N, STMT)
+
 SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[kRegs], const c64* __restrict__ M)
 {
     switch (smask) {
-#define SVMI_DENSE_CASE(S) case S: apply_dense<S>(a, M); break;
+#define SVMI_DENSE_CASE(S) SVMI_CASE(S, apply_dense<S>(a, M))
         SVMI_DENSE_CASE(1) SVMI_DENSE_CASE(2) SVMI_DENSE_CASE(3) SVMI_DENSE_CASE(4)
         SVMI_DENSE_CASE(5) SVMI_DENSE_CASE(6) SVMI_DENSE_CASE(7) SVMI_DENSE_CASE(8)
         SVMI_DENSE_CASE(9) SVMI_DENSE_CASE(10) SVMI_DENSE_CASE(11) SVMI_DENSE_CASE(12)
@@ -23,42 +39,45 @@ SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[kRegs], const c64* __rest
 SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&p)[kRegs], const c64 (&l)[kRegs],
                                           const c64* __restrict__ M)
 {
+    double sum = 0.0;
     switch (smask) {
-#define SVMI_GRAD_CASE(S) case S: return grad_dense<S>(p, l, M);
+#define SVMI_GRAD_CASE(S) SVMI_CASE(S, sum = grad_dense<S>(p, l, M))
         SVMI_GRAD_CASE(1) SVMI_GRAD_CASE(2) SVMI_GRAD_CASE(3) SVMI_GRAD_CASE(4)
         SVMI_GRAD_CASE(5) SVMI_GRAD_CASE(6) SVMI_GRAD_CASE(7) SVMI_GRAD_CASE(8)
         SVMI_GRAD_CASE(9) SVMI_GRAD_CASE(10) SVMI_GRAD_CASE(11) SVMI_GRAD_CASE(12)
         SVMI_GRAD_CASE(13) SVMI_GRAD_CASE(14) SVMI_GRAD_CASE(15)
 #undef SVMI_GRAD_CASE
-    default: return 0.0;
+    default: break;
     }
+    return sum;
 }
 
 SVMI_HD inline void dispatch_x(int s, c64 (&a)[kRegs])
 {
     switch (s) {
-    case 0: apply_x<0>(a); break;
-    case 1: apply_x<1>(a); break;
-    case 2: apply_x<2>(a); break;
-    default: apply_x<3>(a); break;
+        SVMI_CASE(0, apply_x<0>(a))
+        SVMI_CASE(1, apply_x<1>(a))
+        SVMI_CASE(2, apply_x<2>(a))
+        SVMI_CASE(3, apply_x<3>(a))
+    default: break;
     }
 }
 
 SVMI_HD inline void dispatch_cx(int c, int t, c64 (&a)[kRegs])
 {
     switch (c * 4 + t) {
-    case 1: apply_cx<0, 1>(a); break;
-    case 2: apply_cx<0, 2>(a); break;
-    case 3: apply_cx<0, 3>(a); break;
-    case 4: apply_cx<1, 0>(a); break;
-    case 6: apply_cx<1, 2>(a); break;
-    case 7: apply_cx<1, 3>(a); break;
-    case 8: apply_cx<2, 0>(a); break;
-    case 9: apply_cx<2, 1>(a); break;
-    case 11: apply_cx<2, 3>(a); break;
-    case 12: apply_cx<3, 0>(a); break;
-    case 13: apply_cx<3, 1>(a); break;
-    case 14: apply_cx<3, 2>(a); break;
+        SVMI_CASE(1, (apply_cx<0, 1>(a)))
+        SVMI_CASE(2, (apply_cx<0, 2>(a)))
+        SVMI_CASE(3, (apply_cx<0, 3>(a)))
+        SVMI_CASE(4, (apply_cx<1, 0>(a)))
+        SVMI_CASE(6, (apply_cx<1, 2>(a)))
+        SVMI_CASE(7, (apply_cx<1, 3>(a)))
+        SVMI_CASE(8, (apply_cx<2, 0>(a)))
+        SVMI_CASE(9, (apply_cx<2, 1>(a)))
+        SVMI_CASE(11, (apply_cx<2, 3>(a)))
+        SVMI_CASE(12, (apply_cx<3, 0>(a)))
+        SVMI_CASE(13, (apply_cx<3, 1>(a)))
+        SVMI_CASE(14, (apply_cx<3, 2>(a)))
     default: break;
     }
 }
@@ -66,12 +85,12 @@ SVMI_HD inline void dispatch_cx(int c, int t, c64 (&a)[kRegs])
 SVMI_HD inline void dispatch_swap(int s0, int s1, c64 (&a)[kRegs])
 {
     switch (s0 * 4 + s1) {
-    case 1: apply_swap<0, 1>(a); break;
-    case 2: apply_swap<0, 2>(a); break;
-    case 3: apply_swap<0, 3>(a); break;
-    case 6: apply_swap<1, 2>(a); break;
-    case 7: apply_swap<1, 3>(a); break;
-    case 11: apply_swap<2, 3>(a); break;
+        SVMI_CASE(1, (apply_swap<0, 1>(a)))
+        SVMI_CASE(2, (apply_swap<0, 2>(a)))
+        SVMI_CASE(3, (apply_swap<0, 3>(a)))
+        SVMI_CASE(6, (apply_swap<1, 2>(a)))
+        SVMI_CASE(7, (apply_swap<1, 3>(a)))
+        SVMI_CASE(11, (apply_swap<2, 3>(a)))
     default: break;
     }
 }
@@ -79,19 +98,24 @@ SVMI_HD inline void dispatch_swap(int s0, int s1, c64 (&a)[kRegs])
 SVMI_HD inline void dispatch_srn(int s, c64 (&a)[kRegs])
 {
     switch (s) {
-    case 0: apply_srn<0>(a); break;
-    case 1: apply_srn<1>(a); break;
-    case 2: apply_srn<2>(a); break;
-    default: apply_srn<3>(a); break;
+        SVMI_CASE(0, apply_srn<0>(a))
+        SVMI_CASE(1, apply_srn<1>(a))
+        SVMI_CASE(2, apply_srn<2>(a))
+        SVMI_CASE(3, apply_srn<3>(a))
+    default: break;
     }
 }
 
 // Index of a diagonal op's table entry for tile-local index l.
+// (fixed trip count so every index stays static after unrolling)
 SVMI_HD inline uint32_t diag_index(const DOp& op, uint32_t l, uint32_t eidx)
 {
     uint32_t idx = eidx;
-    for (int i = 0; i < op.nq; i++)
-        if (op.qloc[i] >= 0) idx |= ((l >> op.qloc[i]) & 1u) << i;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const int q = op.qloc[i];
+        if (i < op.nq && q >= 0) idx |= ((l >> q) & 1u) << i;
+    }
     return idx;
 }
 
@@ -99,56 +123,69 @@ SVMI_HD inline uint32_t diag_index(const DOp& op, uint32_t l, uint32_t eidx)
 SVMI_HD inline uint32_t diag_ext_index(const DOp& op, uint64_t base)
 {
     uint32_t idx = 0;
-    for (int i = 0; i < op.nq; i++)
-        if (op.qloc[i] < 0) idx |= uint32_t((base >> op.qphys[i]) & 1u) << i;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if (i < op.nq && op.qloc[i] < 0) idx |= uint32_t((base >> op.qphys[i]) & 1u) << i;
     return idx;
+}
+
+// Controlled phase: amp *= ph where every qubit of the op is 1.  The
+// register part of the condition is the uniform mask rsel, the group part
+// one compare per thread, the part outside the tile one compare per tile.
+SVMI_HD inline void apply_cphase(const DOpHdr& op, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+                                 uint64_t base)
+{
+    if ((base & op.emask) != op.emask) return;
+    if ((goff & op.gmask) != op.gmask) return;
+    const c64 ph = coef[op.coef];
+    const uint32_t rsel = op.rsel;
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if ((rsel >> r) & 1u) a[r] = cmul(a[r], ph);
+}
+
+SVMI_HD inline void apply_diag(const DOp& op, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+                               const uint32_t (&off)[kRegs], uint64_t base)
+{
+    const uint32_t e = diag_ext_index(op, base);
+    const c64* tab = coef + op.h.coef;
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) a[r] = cmul(a[r], tab[diag_index(op, goff | off[r], e)]);
 }
 
 // Apply one non-gradient op to one state's register group.
 // goff = tile-local index bits of the group (non-register bits),
 // off[rho] = tile-local index bits of register rho, base = tile base index.
-SVMI_HD inline void exec_state_op(const DOp& op, c64 (&a)[kRegs], const c64* __restrict__ coef,
-                                  uint32_t goff, const uint32_t (&off)[kRegs], uint64_t base)
+// h = the op's header (prefetched), op = the op in memory (diagonal tail).
+SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[kRegs],
+                                  const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
+                                  uint64_t base)
 {
-    switch (op.type) {
-    case DOP_DENSE: dispatch_dense(op.smask, a, coef + op.coef); break;
-    case DOP_X: dispatch_x(op.s0, a); break;
-    case DOP_CX: dispatch_cx(op.s0, op.s1, a); break;
-    case DOP_SWAP: dispatch_swap(op.s0, op.s1, a); break;
-    case DOP_SRN: dispatch_srn(op.s0, a); break;
-    case DOP_CPHASE: {
-        if ((base & op.emask) != op.emask) break;
-        const c64 ph = coef[op.coef];
-        const uint32_t lm = op.lmask;
-#pragma unroll
-        for (int r = 0; r < kRegs; r++)
-            if (((goff | off[r]) & lm) == lm) a[r] = cmul(a[r], ph);
-        break;
-    }
-    case DOP_DIAG: {
-        const uint32_t e = diag_ext_index(op, base);
-        const c64* tab = coef + op.coef;
-#pragma unroll
-        for (int r = 0; r < kRegs; r++) a[r] = cmul(a[r], tab[diag_index(op, goff | off[r], e)]);
-        break;
-    }
+    switch (h.type) {
+        SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, dispatch_dense(h.smask, a, coef + h.coef))
+        SVMI_CASE_ID(DOP_X, DOP_X + 100, dispatch_x(h.s0, a))
+        SVMI_CASE_ID(DOP_CX, DOP_CX + 100, dispatch_cx(h.s0, h.s1, a))
+        SVMI_CASE_ID(DOP_SWAP, DOP_SWAP + 100, dispatch_swap(h.s0, h.s1, a))
+        SVMI_CASE_ID(DOP_SRN, DOP_SRN + 100, dispatch_srn(h.s0, a))
+        SVMI_CASE_ID(DOP_CPHASE, DOP_CPHASE + 100, apply_cphase(h, a, coef, goff, base))
+        SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag(*op, a, coef, goff, off, base))
     default: break;
     }
 }
 
 // Gradient op: Re sum conj(lam) K phi over this group.
-SVMI_HD inline double exec_grad_op(const DOp& op, const c64 (&p)[kRegs], const c64 (&l)[kRegs],
-                                   const c64* __restrict__ coef, uint32_t goff,
+SVMI_HD inline double exec_grad_op(const DOpHdr& h, const DOp* __restrict__ op, const c64 (&p)[kRegs],
+                                   const c64 (&l)[kRegs], const c64* __restrict__ coef, uint32_t goff,
                                    const uint32_t (&off)[kRegs], uint64_t base)
 {
-    if (op.type == DOP_GRAD_DENSE) return dispatch_grad_dense(op.smask, p, l, coef + op.coef);
+    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense(h.smask, p, l, coef + h.coef);
     // DOP_GRAD_DIAG
-    const uint32_t e = diag_ext_index(op, base);
-    const c64* tab = coef + op.coef;
+    const uint32_t e = diag_ext_index(*op, base);
+    const c64* tab = coef + h.coef;
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < kRegs; r++) {
-        const c64 v = cmul(tab[diag_index(op, goff | off[r], e)], p[r]);
+        const c64 v = cmul(tab[diag_index(*op, goff | off[r], e)], p[r]);
         s += l[r].re * v.re + l[r].im * v.im;
     }
     return s;
@@ -186,8 +223,8 @@ SVMI_HD inline uint32_t lds_slot(uint32_t l) { return l ^ ((l >> 4) & 15u); }
 SVMI_HD inline uint64_t tile_base(const SweepInfo& sw, uint64_t tile)
 {
     uint64_t base = 0;
-    for (uint32_t e = 0; e < sw.n_ext; e++)
-        base |= ((tile >> e) & 1ull) << sw.ext[e];
+    for (uint64_t m = sw.extmask; m; m &= m - 1, tile >>= 1)
+        base |= (tile & 1ull) << __builtin_ctzll(m);
     return base;
 }
 

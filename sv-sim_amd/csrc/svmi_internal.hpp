@@ -19,7 +19,7 @@
 #include <stddef.h>
 
 #if defined(__HIPCC__)
-#define SVMI_HD __host__ __device__
+#define SVMI_HD __host__ __device__ __attribute__((always_inline))
 #else
 #define SVMI_HD
 #endif
@@ -30,15 +30,24 @@ struct alignas(16) c64 {
     double re, im;
 };
 
-SVMI_HD inline c64 cmul(c64 a, c64 b) { return c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SVMI_FMA(a, b, c) __builtin_fma((a), (b), (c))
+#else
+#define SVMI_FMA(a, b, c) ((a) * (b) + (c))
+#endif
+
+SVMI_HD inline c64 cmul(c64 a, c64 b)
+{
+    return c64{SVMI_FMA(a.re, b.re, -(a.im * b.im)), SVMI_FMA(a.re, b.im, a.im * b.re)};
+}
 SVMI_HD inline c64 cadd(c64 a, c64 b) { return c64{a.re + b.re, a.im + b.im}; }
-// acc + m * v
+// acc + m * v (four FMAs on the device)
 SVMI_HD inline c64 cmac(c64 acc, c64 m, c64 v)
 {
-    acc.re += m.re * v.re;
-    acc.re -= m.im * v.im;
-    acc.im += m.re * v.im;
-    acc.im += m.im * v.re;
+    acc.re = SVMI_FMA(m.re, v.re, acc.re);
+    acc.re = SVMI_FMA(-m.im, v.im, acc.re);
+    acc.im = SVMI_FMA(m.re, v.im, acc.im);
+    acc.im = SVMI_FMA(m.im, v.re, acc.im);
     return acc;
 }
 // conj(a) * b
@@ -69,25 +78,30 @@ enum DOpType : uint16_t {
 // Which state(s) an op applies to in adjoint sweeps.
 enum : uint8_t { DOP_TO_PSI = 1, DOP_TO_LAM = 2, DOP_TO_BOTH = 3 };
 
-// One device op (uniform across the workgroup; read with scalar loads).
-struct DOp {
-    uint16_t type;
-    uint16_t smask;      // DENSE / GRAD_DENSE: register slot mask
-    uint8_t s0, s1;      // X/CX/SWAP/SRN slots
-    uint8_t nq;          // DIAG / CPHASE: number of qubits
-    uint8_t targets;     // adjoint: DOP_TO_* (forward sweeps: DOP_TO_PSI)
+// One device op (uniform across the workgroup).  Every field is a full
+// dword so the kernels fetch it with scalar loads; the kernels prefetch the
+// 40-byte header one op ahead and read the diagonal-op tail on demand.
+struct DOpHdr {
+    uint32_t type;
+    uint32_t smask;      // DENSE / GRAD_DENSE: register slot mask
+    uint32_t s0, s1;     // X/CX/SWAP/SRN slots
     uint32_t coef;       // offset into the coefficient array (c64 units)
     uint32_t grad;       // GRAD_*: accumulator slot within the sweep
-    uint32_t lmask;      // CPHASE: tile-local bits that must be 1
-    uint64_t emask;      // CPHASE/DIAG: physical bits outside the tile that must be 1 (CPHASE)
-    int8_t qloc[5];      // DIAG: tile-local bit of qubit i, or -1 if outside the tile
-    uint8_t qphys[5];    // DIAG: physical position of qubit i
-    uint8_t pad[6];
+    uint32_t rsel;       // CPHASE: registers rho whose register bits satisfy the op
+    uint32_t gmask;      // CPHASE: non-register tile-local bits that must be 1
+    uint64_t emask;      // CPHASE: physical bits outside the tile that must be 1
 };
-static_assert(sizeof(DOp) == 48, "DOp layout");
+struct DOp {
+    DOpHdr h;
+    uint32_t nq;         // DIAG: number of qubits
+    uint32_t targets;    // adjoint: DOP_TO_* (forward sweeps: DOP_TO_PSI)
+    int32_t qloc[5];     // DIAG: tile-local bit of qubit i, or -1 if outside the tile
+    uint32_t qphys[5];   // DIAG: physical position of qubit i
+};
+static_assert(sizeof(DOpHdr) == 40 && sizeof(DOp) == 88, "DOp layout");
 
 struct PhaseInfo {
-    uint8_t r[4];        // register bits (tile-local positions, ascending)
+    uint32_t r[4];       // register bits (tile-local positions, ascending)
     uint32_t op_begin, op_end;
 };
 
@@ -99,7 +113,7 @@ struct SweepInfo {
     uint32_t hitab;      // offset into the hi-offset table (2^(k-c) entries)
     uint32_t n_grad;     // adjoint: accumulators used by this sweep
     uint32_t grad_base;  // adjoint: first global accumulator of this sweep
-    uint8_t ext[40];     // physical positions outside the tile (ascending)
+    uint64_t extmask;    // physical qubits outside the tile
     uint64_t n_tiles;
 };
 

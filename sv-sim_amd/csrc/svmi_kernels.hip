@@ -20,11 +20,11 @@ namespace svmi {
 // ------------------------------------------------------------------ sweeps
 
 template <int NS>
-__global__ void __launch_bounds__(kThreads)
-k_sweep(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw,
-        const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
-        const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
-        double* __restrict__ grad_partials, uint64_t tile_offset)
+__device__ __attribute__((always_inline)) inline void
+sweep_body(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
+           const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
+           const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
+           double* __restrict__ grad_partials, uint64_t tile_offset)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t k = sw.k;
@@ -64,16 +64,21 @@ k_sweep(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw,
             a[r] = t0[lds_slot(goff | off[r])];
             if (NS == 2) b[r] = t1[lds_slot(goff | off[r])];
         }
+        // op headers are fetched one op ahead so the scalar-load latency of
+        // the next op overlaps the arithmetic of the current one
+        DOpHdr nxt;
+        if (ph.op_begin < ph.op_end) nxt = ops[ph.op_begin].h;
         for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
-            const DOp op = ops[o];
-            if (NS == 2 && (op.type == DOP_GRAD_DENSE || op.type == DOP_GRAD_DIAG)) {
-                double v = exec_grad_op(op, a, b, coef, goff, off, base);
+            const DOpHdr h = nxt;
+            if (o + 1 < ph.op_end) nxt = ops[o + 1].h;
+            if (NS == 2 && (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG)) {
+                double v = exec_grad_op(h, ops + o, a, b, coef, goff, off, base);
                 const int width = nt < 64 ? int(nt) : 64;
                 for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
-                if ((tid & 63u) == 0) gacc[op.grad * nwaves + (tid >> 6)] += v;
+                if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
             } else {
-                exec_state_op(op, a, coef, goff, off, base);
-                if (NS == 2) exec_state_op(op, b, coef, goff, off, base);
+                exec_state_op(h, ops + o, a, coef, goff, off, base);
+                if (NS == 2) exec_state_op(h, ops + o, b, coef, goff, off, base);
             }
         }
 #pragma unroll
@@ -102,6 +107,24 @@ k_sweep(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw,
 
 static size_t g_max_smem = 64 * 1024;  // dynamic LDS the sweep kernels may use
 
+// One state: 64 KiB tile -> 2 workgroups (8 waves) per CU, so 256 VGPRs.
+__global__ void __launch_bounds__(kThreads, 2)
+k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
+         const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
+         uint64_t tile_offset)
+{
+    sweep_body<1>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset);
+}
+
+// Two states (adjoint): one wave per SIMD, registers for both groups.
+__global__ void __launch_bounds__(kThreads, 1)
+k_sweep2(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
+         const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
+         double* __restrict__ grad_partials, uint64_t tile_offset)
+{
+    sweep_body<2>(s0, s1, sw, phases, ops, coef, hitab, grad_partials, tile_offset);
+}
+
 size_t sweep_smem_bytes(const SweepInfo& sw, int ns)
 {
     const size_t nt = size_t(1) << (sw.k - kRegBits);
@@ -120,11 +143,11 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
     for (uint64_t t0 = 0; t0 < sw.n_tiles; t0 += max_grid) {
         const uint64_t cnt = sw.n_tiles - t0 < max_grid ? sw.n_tiles - t0 : max_grid;
         if (ns == 1)
-            hipLaunchKernelGGL(k_sweep<1>, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases,
-                               ops, coef, hitab, grad_partials, t0);
+            hipLaunchKernelGGL(k_sweep1, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, sw, phases, ops, coef,
+                               hitab, t0);
         else
-            hipLaunchKernelGGL(k_sweep<2>, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases,
-                               ops, coef, hitab, grad_partials, t0);
+            hipLaunchKernelGGL(k_sweep2, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases, ops,
+                               coef, hitab, grad_partials, t0);
     }
     return hipGetLastError();
 }
@@ -144,10 +167,10 @@ hipError_t init_sweep_kernels()
     size_t want = size_t(optin);
     if (want > 160 * 1024) want = 160 * 1024;
     if (want > 64 * 1024) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep<1>),
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep<2>),
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep2),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e != hipSuccess) {
             (void)hipGetLastError();

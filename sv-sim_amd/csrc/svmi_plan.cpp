@@ -12,6 +12,7 @@
 #include "svmi_plan.hpp"
 
 #include <algorithm>
+#include <deque>
 #include <cstring>
 #include <sstream>
 
@@ -113,10 +114,10 @@ struct Encoder {
         DOp d;
         std::memset(&d, 0, sizeof(d));
         d.targets = prog.n_states == 2 ? DOP_TO_BOTH : DOP_TO_PSI;
-        d.grad = grad_slot;
+        d.h.grad = grad_slot;
         if (op.kind == LOp::SRN) {
-            d.type = DOP_SRN;
-            d.s0 = uint8_t(slot_of[tloc[op.q[0]]]);
+            d.h.type = DOP_SRN;
+            d.h.s0 = uint32_t(slot_of[tloc[op.q[0]]]);
             prog.ops.push_back(d);
             return;
         }
@@ -125,20 +126,24 @@ struct Encoder {
             bool cphase = true;
             for (size_t r = 0; r + 1 < D; r++)
                 if (op.m[r].re != 1.0 || op.m[r].im != 0.0) cphase = false;
-            d.nq = uint8_t(op.k);
+            d.nq = uint32_t(op.k);
+            uint32_t rbits = 0;   // register slots the op needs set
             for (int i = 0; i < op.k; i++) {
                 const int l = tloc[op.q[i]];
-                d.qloc[i] = int8_t(l);
-                d.qphys[i] = uint8_t(op.q[i]);
-                if (l >= 0) d.lmask |= 1u << l;
-                else d.emask |= uint64_t(1) << op.q[i];
+                d.qloc[i] = int32_t(l);
+                d.qphys[i] = uint32_t(op.q[i]);
+                if (l < 0) d.h.emask |= uint64_t(1) << op.q[i];
+                else if (slot_of[l] >= 0) rbits |= 1u << slot_of[l];
+                else d.h.gmask |= 1u << l;
             }
+            for (int r = 0; r < kRegs; r++)
+                if ((uint32_t(r) & rbits) == rbits) d.h.rsel |= 1u << r;
             if (cphase && !s.grad) {
-                d.type = DOP_CPHASE;
-                d.coef = push_coef(std::vector<c64>{op.m[D - 1]});
+                d.h.type = DOP_CPHASE;
+                d.h.coef = push_coef(std::vector<c64>{op.m[D - 1]});
             } else {
-                d.type = s.grad ? DOP_GRAD_DIAG : DOP_DIAG;
-                d.coef = push_coef(op.m);
+                d.h.type = s.grad ? DOP_GRAD_DIAG : DOP_DIAG;
+                d.h.coef = push_coef(op.m);
             }
             prog.ops.push_back(d);
             return;
@@ -150,8 +155,8 @@ struct Encoder {
             if (op.k == 1) {
                 const int src[2] = {1, 0};
                 if (is_perm(op, src)) {
-                    d.type = DOP_X;
-                    d.s0 = uint8_t(slots[0]);
+                    d.h.type = DOP_X;
+                    d.h.s0 = uint32_t(slots[0]);
                     prog.ops.push_back(d);
                     return;
                 }
@@ -161,19 +166,19 @@ struct Encoder {
                 const int cx10[4] = {0, 1, 3, 2};   // control q1, target q0
                 const int sw[4] = {0, 2, 1, 3};
                 if (is_perm(op, cx01)) {
-                    d.type = DOP_CX; d.s0 = uint8_t(slots[0]); d.s1 = uint8_t(slots[1]);
+                    d.h.type = DOP_CX; d.h.s0 = uint32_t(slots[0]); d.h.s1 = uint32_t(slots[1]);
                     prog.ops.push_back(d);
                     return;
                 }
                 if (is_perm(op, cx10)) {
-                    d.type = DOP_CX; d.s0 = uint8_t(slots[1]); d.s1 = uint8_t(slots[0]);
+                    d.h.type = DOP_CX; d.h.s0 = uint32_t(slots[1]); d.h.s1 = uint32_t(slots[0]);
                     prog.ops.push_back(d);
                     return;
                 }
                 if (is_perm(op, sw)) {
-                    d.type = DOP_SWAP;
-                    d.s0 = uint8_t(std::min(slots[0], slots[1]));
-                    d.s1 = uint8_t(std::max(slots[0], slots[1]));
+                    d.h.type = DOP_SWAP;
+                    d.h.s0 = uint32_t(std::min(slots[0], slots[1]));
+                    d.h.s1 = uint32_t(std::max(slots[0], slots[1]));
                     prog.ops.push_back(d);
                     return;
                 }
@@ -195,11 +200,152 @@ struct Encoder {
         };
         for (size_t r = 0; r < D; r++)
             for (size_t c = 0; c < D; c++) M[remap(r) * D + remap(c)] = op.m[r * D + c];
-        d.type = s.grad ? DOP_GRAD_DENSE : DOP_DENSE;
-        d.smask = 0;
-        for (int i = 0; i < op.k; i++) d.smask |= uint16_t(1u << slots[i]);
-        d.coef = push_coef(M);
+        d.h.type = s.grad ? DOP_GRAD_DENSE : DOP_DENSE;
+        d.h.smask = 0;
+        for (int i = 0; i < op.k; i++) d.h.smask |= 1u << slots[i];
+        d.h.coef = push_coef(M);
         prog.ops.push_back(d);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Register-run merging.  Inside a phase, consecutive ops that act only on the
+// 4 register slots (dense ops, and diagonal ops whose qubits are all register
+// bits) are multiplied into one dense op on the union of their slots when
+// that costs no more than running them one by one; every op run costs a fixed
+// decode overhead on top of its arithmetic.  Ops that are not register-local
+// (diagonals on other bits, gradient reads) commute with the run when they
+// share no qubit with it and are emitted ahead of it; otherwise the run is
+// flushed first.  SRN (real-linear) always flushes.
+// ---------------------------------------------------------------------------
+constexpr long kOpOverhead = 8;  // per-op decode cost in FMA per amplitude
+
+long reg_cost(const LOp& op)
+{
+    if (op.kind == LOp::DIAG) return 2;
+    return 4L << op.k;
+}
+
+// matrix of `op` on the slot set S (matrix bit j <-> j-th lowest slot of S)
+void embed_in_slots(const LOp& op, const int* tloc, const int* slot_of, uint32_t S, std::vector<c64>& M)
+{
+    int pos_of_slot[kRegBits];
+    int q = 0;
+    for (int s = 0; s < kRegBits; s++)
+        if ((S >> s) & 1) pos_of_slot[s] = q++;
+    const size_t D = size_t(1) << q;
+    int bitpos[5];
+    uint32_t opmask = 0;
+    for (int i = 0; i < op.k; i++) {
+        bitpos[i] = pos_of_slot[slot_of[tloc[op.q[i]]]];
+        opmask |= 1u << bitpos[i];
+    }
+    const size_t Dop = size_t(1) << op.k;
+    std::vector<c64> A;
+    lop_to_dense(op, A);
+    M.assign(D * D, c64{0.0, 0.0});
+    for (size_t c = 0; c < D; c++) {
+        size_t cl = 0;
+        for (int i = 0; i < op.k; i++)
+            if ((c >> bitpos[i]) & 1) cl |= size_t(1) << i;
+        const size_t rest = c & ~size_t(opmask);
+        for (size_t rl = 0; rl < Dop; rl++) {
+            size_t r = rest;
+            for (int i = 0; i < op.k; i++)
+                if ((rl >> i) & 1) r |= size_t(1) << bitpos[i];
+            M[r * D + c] = A[rl * Dop + cl];
+        }
+    }
+}
+
+struct RunMerger {
+    const int* tloc;
+    const int* slot_of;
+    const int* phys_of_slot;
+    std::deque<LOp>& owned;
+    std::vector<const SOp*>& out;
+    std::deque<SOp>& sowned;
+    std::vector<const SOp*> run;
+    uint64_t run_mask = 0;
+
+    bool reg_local(const SOp* s) const
+    {
+        if (s->grad || s->op->kind == LOp::SRN) return false;
+        for (int i = 0; i < s->op->k; i++) {
+            const int l = tloc[s->op->q[i]];
+            if (l < 0 || slot_of[l] < 0) return false;
+        }
+        return true;
+    }
+
+    void flush()
+    {
+        if (run.size() <= 1) {
+            for (const SOp* s : run) out.push_back(s);
+            run.clear();
+            run_mask = 0;
+            return;
+        }
+        uint32_t S = 0;
+        long separate = 0;
+        for (const SOp* s : run) {
+            for (int i = 0; i < s->op->k; i++) S |= 1u << slot_of[tloc[s->op->q[i]]];
+            separate += reg_cost(*s->op) + kOpOverhead;
+        }
+        const int q = __builtin_popcount(S);
+        if ((4L << q) + kOpOverhead > separate) {
+            for (const SOp* s : run) out.push_back(s);
+        } else {
+            const size_t D = size_t(1) << q;
+            std::vector<c64> B(D * D, c64{0.0, 0.0}), E, T(D * D);
+            for (size_t i = 0; i < D; i++) B[i * D + i] = c64{1.0, 0.0};
+            for (const SOp* s : run) {
+                embed_in_slots(*s->op, tloc, slot_of, S, E);
+                for (size_t r = 0; r < D; r++)
+                    for (size_t c = 0; c < D; c++) {
+                        c64 acc{0.0, 0.0};
+                        for (size_t j = 0; j < D; j++) acc = cmac(acc, E[r * D + j], B[j * D + c]);
+                        T[r * D + c] = acc;
+                    }
+                B.swap(T);
+            }
+            LOp m;
+            m.k = q;
+            int qi = 0;
+            for (int sl = 0; sl < kRegBits; sl++)
+                if ((S >> sl) & 1) m.q[qi++] = phys_of_slot[sl];
+            bool diag = true;
+            for (size_t r = 0; r < D && diag; r++)
+                for (size_t c = 0; c < D; c++)
+                    if (r != c && (B[r * D + c].re != 0.0 || B[r * D + c].im != 0.0)) {
+                        diag = false;
+                        break;
+                    }
+            if (diag) {
+                m.kind = LOp::DIAG;
+                m.m.resize(D);
+                for (size_t r = 0; r < D; r++) m.m[r] = B[r * D + r];
+            } else {
+                m.kind = LOp::DENSE;
+                m.m.swap(B);
+            }
+            owned.push_back(std::move(m));
+            sowned.push_back(make_sop(&owned.back(), false, -1));
+            out.push_back(&sowned.back());
+        }
+        run.clear();
+        run_mask = 0;
+    }
+
+    void add(const SOp* s)
+    {
+        if (reg_local(s)) {
+            run.push_back(s);
+            run_mask |= s->mask;
+            return;
+        }
+        if ((s->mask & run_mask) || s->op->kind == LOp::SRN) flush();
+        out.push_back(s);
     }
 };
 
@@ -221,6 +367,8 @@ void emit_standalone(Program& prog, const SOp& s)
 void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T, int n, int c)
 {
     Encoder enc(prog);
+    std::deque<LOp> owned;     // merged ops (live until encoded)
+    std::deque<SOp> sowned;
     int tpos[64];
     int tloc[64];
     int k = 0;
@@ -238,8 +386,12 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     sw.phase_begin = uint32_t(prog.phases.size());
     sw.hitab = uint32_t(prog.hitab.size());
     sw.n_ext = 0;
+    sw.extmask = 0;
     for (int p = 0; p < n; p++)
-        if (!((T >> p) & 1)) sw.ext[sw.n_ext++] = uint8_t(p);
+        if (!((T >> p) & 1)) {
+            sw.extmask |= uint64_t(1) << p;
+            sw.n_ext++;
+        }
     sw.n_tiles = uint64_t(1) << (n - k);
     for (uint64_t h = 0; h < (uint64_t(1) << (k - c)); h++) {
         uint64_t off = 0;
@@ -249,33 +401,76 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     }
     sw.grad_base = prog.n_grad;
 
-    // phase scheduling
+    // phase scheduling: pick the register bits R of each phase by a small
+    // search that maximises the work the phase can take, then take it
     std::vector<const SOp*> remaining = taken;
     uint32_t n_grad_sweep = 0;
-    while (!remaining.empty()) {
-        uint32_t R = 0;  // local-bit mask
+    auto local_mask = [&](const SOp* s) {
+        uint32_t lm = 0;
+        for (int i = 0; i < s->op->k; i++) {
+            const int l = tloc[s->op->q[i]];
+            if (l >= 0) lm |= 1u << l;
+        }
+        return lm;
+    };
+    auto all_local = [&](const SOp* s) {
+        for (int i = 0; i < s->op->k; i++)
+            if (tloc[s->op->q[i]] < 0) return false;
+        return true;
+    };
+    // ops a phase with fixed register bits R takes; returns a score
+    auto take_with = [&](uint32_t R, std::vector<const SOp*>* ph, std::vector<const SOp*>* rest) {
         uint64_t blocked = 0;
-        std::vector<const SOp*> ph, rest;
+        long score = 0;
         for (const SOp* s : remaining) {
             if (s->dep & blocked) {
                 blocked |= s->dep;
-                rest.push_back(s);
+                if (rest) rest->push_back(s);
                 continue;
             }
+            const uint32_t lm = local_mask(s);
             if (s->diag) {
-                ph.push_back(s);
+                if (ph) ph->push_back(s);
+                if (all_local(s) && (lm & ~R) == 0) score += 6;          // merges into a run
+                else if (lm & R) score -= 6;                             // splits a run
                 continue;
             }
-            uint32_t lm = 0;
-            for (int i = 0; i < s->op->k; i++) lm |= 1u << tloc[s->op->q[i]];
-            if (popc(R | lm) <= kRegBits) {
-                R |= lm;
-                ph.push_back(s);
+            if ((lm & ~R) == 0) {
+                if (ph) ph->push_back(s);
+                score += reg_cost(*s->op);
             } else {
                 blocked |= s->dep;
-                rest.push_back(s);
+                if (rest) rest->push_back(s);
             }
         }
+        return score;
+    };
+    while (!remaining.empty()) {
+        uint32_t R = 0;  // local-bit mask
+        for (const SOp* s : remaining)
+            if (!s->diag) {
+                R = local_mask(s);
+                break;
+            }
+        // candidate bits: tile bits of the first pending ops
+        uint32_t cand = 0;
+        for (size_t i = 0; i < remaining.size() && i < 96; i++) cand |= local_mask(remaining[i]);
+        while (popc(R) < kRegBits) {
+            long best = -(1L << 60);
+            int best_b = -1;
+            for (int b = 0; b < k; b++) {
+                if (((R >> b) & 1) || !((cand >> b) & 1)) continue;
+                const long sc = take_with(R | (1u << b), nullptr, nullptr);
+                if (sc > best) {
+                    best = sc;
+                    best_b = b;
+                }
+            }
+            if (best_b < 0) break;
+            R |= 1u << best_b;
+        }
+        std::vector<const SOp*> ph, rest;
+        take_with(R, &ph, &rest);
         // pad R to 4 bits with the lowest free tile bits above the low qubits first
         for (int l = 0; l < k && popc(R) < kRegBits; l++)
             if (!((R >> l) & 1) && l >= c) R |= 1u << l;
@@ -288,11 +483,19 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         int ns = 0;
         for (int l = 0; l < k; l++)
             if ((R >> l) & 1) {
-                pi.r[ns] = uint8_t(l);
+                pi.r[ns] = uint32_t(l);
                 slot_of[l] = ns++;
             }
+        int phys_of_slot[kRegBits];
+        for (int sl = 0; sl < kRegBits; sl++) phys_of_slot[sl] = tpos[pi.r[sl]];
+        std::vector<const SOp*> merged;
+        {
+            RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged, sowned, {}, 0};
+            for (const SOp* s : ph) rm.add(s);
+            rm.flush();
+        }
         pi.op_begin = uint32_t(prog.ops.size());
-        for (const SOp* s : ph) {
+        for (const SOp* s : merged) {
             uint32_t gslot = 0;
             if (s->grad) {
                 gslot = n_grad_sweep++;
@@ -445,8 +648,24 @@ std::string describe_program(const Program& prog)
             for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++)
                 no += prog.phases[p].op_end - prog.phases[p].op_begin;
             ss << no << " ext=[";
-            for (uint32_t e = 0; e < sw.n_ext; e++) ss << (e ? "," : "") << int(sw.ext[e]);
+            bool first = true;
+            for (int p = 0; p < 64; p++)
+                if ((sw.extmask >> p) & 1) {
+                    ss << (first ? "" : ",") << p;
+                    first = false;
+                }
             ss << "]\n";
+            for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
+                const PhaseInfo& ph = prog.phases[p];
+                ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3] << " :";
+                for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
+                    const DOp& d = prog.ops[o];
+                    static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG"};
+                    ss << " " << nm[d.h.type];
+                    if (d.h.type == DOP_DENSE || d.h.type == DOP_GRAD_DENSE) ss << __builtin_popcount(d.h.smask);
+                }
+                ss << "\n";
+            }
         } else {
             const StandaloneOp& so = prog.standalone[st.index];
             ss << "  standalone kind=" << so.kind << " q=" << so.q << "\n";
