@@ -8,7 +8,12 @@ value = gates/s; roofline = the tiled sweep kernel's algorithmic bytes
 launch duration, timed with HIP events on the engine's stream; cpu_baseline =
 the CPU oracle (oracle/, OpenMP) on a bounded prefix of the same circuit.
 
-  python bench.py [--steps K] [--warmup W] [--extra] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--quick] [--no-cpu]
+
+The "extra" object reports the other workloads: the 33-qubit QFT circuit
+(gates/s, sweep GB/s), the adjoint gradient of the 20-qubit 320-parameter
+ansatz (time vs one forward pass and vs GPU parameter shift) and 512 batched
+ansatz gradients vs a per-circuit loop.
 """
 import argparse
 import json
@@ -60,6 +65,98 @@ def bench_u3_brickwall(n, steps, warmup):
             "passes": passes, "sample_s": ts, "total_prob": total}
 
 
+def bench_qft(n, steps, seed=2033):
+    """QFT-structured circuit on a seeded random input state (on device)."""
+    import svmi
+    from svmi import workloads as W
+    gates = W.qft(n=n)
+    e = svmi.Engine(n)
+    e.set_random_state(seed)
+    e.upload(gates)
+    e.run()
+    e.set_option(svmi.OPT_PROFILE, 1)
+    t0 = time.perf_counter()
+    sweep_ms, sweep_n = 0.0, 0
+    for _ in range(steps):
+        e.run()
+        st = e.stats()
+        sweep_ms += st["sweep_ms"]
+        sweep_n += st["sweep_launches"]
+    dt = (time.perf_counter() - t0) / steps
+    passes = e.stats()["sweeps"]
+    e.close()
+    avg = sweep_ms / max(sweep_n, 1) / 1e3
+    return {"qubits": n, "gates": len(gates), "ms_per_circuit": dt * 1e3, "gates_per_s": len(gates) / dt,
+            "passes": passes, "sweep_GBps": 32.0 * (1 << n) / avg / 1e9}
+
+
+def bench_adjoint(n=20, layers=8, reps=5):
+    """<H> and dE/dtheta of the hardware-efficient ansatz (320 parameters)
+    by the adjoint sweep, against one forward pass and against GPU
+    parameter shift (2 forward passes + <H> per parameter)."""
+    import svmi
+    from svmi import workloads as W
+    gates, slots, P, terms = W.hea_ising(n=n, layers=layers)
+    e = svmi.Engine(n)
+    e.upload(gates)
+    fw, adj = [], []
+    for _ in range(reps):
+        e.reset_state()
+        t0 = time.perf_counter()
+        e.run()
+        fw.append(time.perf_counter() - t0)
+    for _ in range(reps):
+        e.reset_state()
+        t0 = time.perf_counter()
+        E, g = e.adjoint(terms, slots, P)
+        adj.append(time.perf_counter() - t0)
+    names = ("theta", "phi", "lambda")
+    t0 = time.perf_counter()
+    ps = np.zeros(P)
+    for gi in range(len(gates)):
+        for s_ in range(3):
+            p = slots[gi, s_]
+            if p < 0:
+                continue
+            v = []
+            for sh in (np.pi / 2, -np.pi / 2):
+                G2 = gates.copy()
+                G2[gi][names[s_]] += sh
+                e.reset_state()
+                e.upload(G2)
+                e.run()
+                v.append(e.expval(terms))
+            ps[p] = 0.5 * (v[0] - v[1])
+    t_ps = time.perf_counter() - t0
+    e.close()
+    f, a = float(np.median(fw)), float(np.median(adj))
+    return {"qubits": n, "params": P, "gates": len(gates), "forward_ms": f * 1e3, "adjoint_ms": a * 1e3,
+            "adjoint_over_forward": a / f, "param_shift_ms": t_ps * 1e3, "param_shift_over_adjoint": t_ps / a,
+            "max_abs_diff_vs_param_shift": float(np.abs(g - ps).max())}
+
+
+def bench_batch(count=512):
+    """Expectation + adjoint gradient of many uneven ansatz circuits: one
+    batched call against a per-circuit loop on the same GPU."""
+    import svmi
+    from svmi import workloads as W
+    items = W.batch_ising(count=count)
+    t0 = time.perf_counter()
+    ev, grads = svmi.batch_adjoint(items)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ev2 = np.zeros(len(items))
+    diff = 0.0
+    for i, it in enumerate(items):
+        with svmi.Engine(it["n"]) as e:
+            e.upload(it["gates"])
+            ev2[i], g = e.adjoint(it["terms"], it["param_slots"], it["n_params"])
+            diff = max(diff, float(np.abs(g - grads[i]).max()))
+    t_loop = time.perf_counter() - t0
+    return {"circuits": len(items), "batched_ms": t_batch * 1e3, "loop_ms": t_loop * 1e3,
+            "speedup": t_loop / t_batch, "max_abs_diff": max(diff, float(np.abs(ev - ev2).max()))}
+
+
 def cpu_baseline_u3(n, budget_s=12.0, threads=None):
     """Oracle (OpenMP C restatement of the reference CPU engine) on the first
     gates of the same circuit, in chunks until ~budget_s of CPU work."""
@@ -88,7 +185,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extra", action="store_true", help="also run the QFT, adjoint and batch workloads")
+    ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine runs on a single device)",
@@ -103,6 +200,17 @@ def main():
     achieved = bytes_per_sweep / avg_sweep_s / 1e9
     log(f"[bench] {ms:.1f} ms/step, {r['passes']} passes/step, sweep avg {avg_sweep_s*1e3:.3f} ms, "
         f"{achieved:.0f} GB/s, sampling 4096 shots {r['sample_s']*1e3:.1f} ms")
+    extra = {}
+    if not a.quick:
+        log("[bench] qft n=33 on a seeded random input")
+        extra["qft_n33"] = bench_qft(33, steps=2)
+        log(f"[bench] {extra['qft_n33']}")
+        log("[bench] adjoint gradient, 20-qubit ansatz, 320 parameters")
+        extra["adjoint_hea_n20"] = bench_adjoint()
+        log(f"[bench] {extra['adjoint_hea_n20']}")
+        log("[bench] 512 batched ansatz gradients vs per-circuit loop")
+        extra["batch_ising_512"] = bench_batch()
+        log(f"[bench] {extra['batch_ising_512']}")
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, OpenMP)")
@@ -128,6 +236,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(out), flush=True)
 

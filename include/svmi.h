@@ -108,8 +108,18 @@ int svmi_num_qubits(svmi_handle h);
  *      sv_real/sv_imag members (svsim_cpu_omp.hpp:576-577). Host SoA arrays of
  *      2^n doubles each. */
 svmi_status svmi_reset_state(svmi_handle h);
+/* Seeded random state (no reference counterpart): amplitude x is
+ * (g0 + i g1) / norm with (g0, g1) = Box-Muller of the two uniforms
+ * u = ((splitmix64(seed + 2x + k) >> 11) + 1) * 2^-53, k = 1, 2 (u0 -> radius,
+ * u1 -> angle); norm makes the state unit length.  Generated on the device,
+ * so 2^33-amplitude states need no host copy. */
+svmi_status svmi_set_random_state(svmi_handle h, uint64_t seed);
 svmi_status svmi_set_state(svmi_handle h, const double* re, const double* im);
 svmi_status svmi_get_state(svmi_handle h, double* re, double* im);
+/* Amplitudes at the given basis indices (no reference counterpart; reads a
+ * few amplitudes of a large state without copying all of it). */
+svmi_status svmi_get_amplitudes(svmi_handle h, const uint64_t* idx, size_t count,
+                                double* re, double* im);
 
 /* ---- circuit: append+upload / clear_circuit (svsim_cpu_omp.hpp:242-275,
  *      321-338).  upload validates (qubit bounds, distinct qubits, opcode),

@@ -393,6 +393,24 @@ svmi_status svmi_reset_state(svmi_handle h)
     return SVMI_OK;
 }
 
+svmi_status svmi_set_random_state(svmi_handle h, uint64_t seed)
+{
+    SVMI_TRY(use_device(h));
+    const uint64_t dim_user = 1ull << h->n;
+    if (h->n_int != h->n) SVMI_HIP(launch_fill_basis(h->stream, h->sv, h->dim));
+    const uint32_t grid = random_state_grid(dim_user);
+    SVMI_TRY(h->scratch[2].ensure(size_t(grid) * sizeof(double)));
+    SVMI_HIP(launch_random_state(h->stream, h->sv, dim_user, seed, h->scratch[2].as<double>()));
+    std::vector<double> part(grid);
+    SVMI_HIP(hipMemcpyAsync(part.data(), h->scratch[2].p, grid * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    double norm2 = 0.0;
+    for (double v : part) norm2 += v;
+    SVMI_HIP(launch_scale(h->stream, h->sv, dim_user, 1.0 / std::sqrt(norm2)));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    return SVMI_OK;
+}
+
 svmi_status svmi_set_state(svmi_handle h, const double* re, const double* im)
 {
     SVMI_TRY(use_device(h));
@@ -442,6 +460,28 @@ svmi_status svmi_get_state(svmi_handle h, double* re, double* im)
     }
     (void)hipHostFree(stage);
     return st;
+}
+
+svmi_status svmi_get_amplitudes(svmi_handle h, const uint64_t* idx, size_t count, double* re, double* im)
+{
+    SVMI_TRY(use_device(h));
+    if (count && (!idx || !re || !im)) return fail(SVMI_ERR_ARG, "null argument");
+    const uint64_t dim_user = 1ull << h->n;
+    for (size_t i = 0; i < count; i++)
+        if (idx[i] >= dim_user) return fail(SVMI_ERR_ARG, "amplitude index out of range");
+    if (count == 0) return SVMI_OK;
+    SVMI_TRY(h->scratch[0].ensure(count * sizeof(uint64_t)));
+    SVMI_TRY(h->scratch[4].ensure(count * sizeof(c64)));
+    SVMI_HIP(hipMemcpyAsync(h->scratch[0].p, idx, count * sizeof(uint64_t), hipMemcpyHostToDevice, h->stream));
+    SVMI_HIP(launch_gather(h->stream, h->sv, h->dim, h->scratch[0].as<uint64_t>(), count, h->scratch[4].as<c64>()));
+    std::vector<c64> out(count);
+    SVMI_HIP(hipMemcpyAsync(out.data(), h->scratch[4].p, count * sizeof(c64), hipMemcpyDeviceToHost, h->stream));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < count; i++) {
+        re[i] = out[i].re;
+        im[i] = out[i].im;
+    }
+    return SVMI_OK;
 }
 
 svmi_status svmi_upload(svmi_handle h, const svmi_gate* gates, size_t n_gates)

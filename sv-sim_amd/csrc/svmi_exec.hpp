@@ -156,6 +156,30 @@ SVMI_HD inline void apply_diag(const DOp& op, c64 (&a)[kRegs], const c64* __rest
 // Apply one non-gradient op to one state's register group.
 // goff = tile-local index bits of the group (non-register bits),
 // off[rho] = tile-local index bits of register rho, base = tile base index.
+// Product of controlled phases (h.s0 entries) applied to registers h.rsel.
+SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+                                   uint64_t base)
+{
+    const PhaseEntry* pe = reinterpret_cast<const PhaseEntry*>(coef + h.coef);
+    double pr = 1.0, pi = 0.0;
+    const uint32_t cnt = h.s0;
+#pragma unroll 4
+    for (uint32_t e = 0; e < cnt; e++) {
+        const PhaseEntry E = pe[e];
+        if ((base & E.emask) != E.emask) continue;
+        const bool on = (goff & E.gmask) == E.gmask;
+        const double tr = SVMI_FMA(pr, E.ph_re, -(pi * E.ph_im));
+        const double ti = SVMI_FMA(pr, E.ph_im, pi * E.ph_re);
+        pr = on ? tr : pr;
+        pi = on ? ti : pi;
+    }
+    const c64 ph{pr, pi};
+    const uint32_t rsel = h.rsel;
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if ((rsel >> r) & 1u) a[r] = cmul(a[r], ph);
+}
+
 // h = the op's header (prefetched), op = the op in memory (diagonal tail).
 SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[kRegs],
                                   const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
@@ -169,6 +193,7 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_CASE_ID(DOP_SRN, DOP_SRN + 100, dispatch_srn(h.s0, a))
         SVMI_CASE_ID(DOP_CPHASE, DOP_CPHASE + 100, apply_cphase(h, a, coef, goff, base))
         SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag(*op, a, coef, goff, off, base))
+        SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset(h, a, coef, goff, base))
     default: break;
     }
 }

@@ -73,7 +73,19 @@ enum DOpType : uint16_t {
     DOP_DIAG = 7,    // amp *= coef[idx], idx from the op's nq qubits
     DOP_GRAD_DENSE = 8,  // adjoint: acc[grad] += Re sum conj(lam) (coef-matrix phi) on slots
     DOP_GRAD_DIAG = 9,   // adjoint: acc[grad] += Re sum conj(lam) coef[idx] phi
+    DOP_PHASESET = 10,   // product of controlled phases, applied to registers rsel
 };
+
+// One controlled phase of a DOP_PHASESET (2 c64 slots in the coefficient
+// array): phase `ph` applies where the group bits gmask and the bits emask
+// outside the tile are all 1 (the register condition is the op's rsel).
+struct PhaseEntry {
+    uint32_t gmask;
+    uint32_t pad;
+    uint64_t emask;
+    double ph_re, ph_im;
+};
+static_assert(sizeof(PhaseEntry) == 32, "PhaseEntry layout");
 
 // Which state(s) an op applies to in adjoint sweeps.
 enum : uint8_t { DOP_TO_PSI = 1, DOP_TO_LAM = 2, DOP_TO_BOTH = 3 };

@@ -302,6 +302,77 @@ __global__ void __launch_bounds__(256) k_copy(c64* __restrict__ d, const c64* __
         d[x] = s[x];
 }
 
+// splitmix64 finaliser (Steele, Lea & Flood 2014)
+__device__ inline uint64_t splitmix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256)
+k_random_state(c64* __restrict__ s, uint64_t dim, uint64_t seed, double* __restrict__ partials)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; x < dim;
+         x += uint64_t(gridDim.x) * blockDim.x) {
+        const double u0 = double((splitmix64(seed + 2 * x + 1) >> 11) + 1) * 0x1p-53;
+        const double u1 = double((splitmix64(seed + 2 * x + 2) >> 11) + 1) * 0x1p-53;
+        const double r = sqrt(-2.0 * log(u0));
+        double sn, cs;
+        sincospi(2.0 * u1, &sn, &cs);
+        const c64 v{r * cs, r * sn};
+        s[x] = v;
+        acc += v.re * v.re + v.im * v.im;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_scale(c64* __restrict__ s, uint64_t dim, double f)
+{
+    for (uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; x < dim;
+         x += uint64_t(gridDim.x) * blockDim.x) {
+        s[x].re *= f;
+        s[x].im *= f;
+    }
+}
+
+uint32_t random_state_grid(uint64_t dim) { return grid_for(dim, 256) > 4096 ? 4096 : grid_for(dim, 256); }
+
+hipError_t launch_random_state(hipStream_t st, c64* s, uint64_t dim, uint64_t seed, double* partials)
+{
+    hipLaunchKernelGGL(k_random_state, dim3(random_state_grid(dim)), dim3(256), 0, st, s, dim, seed, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(hipStream_t st, c64* s, uint64_t dim, double f)
+{
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(dim, 256)), dim3(256), 0, st, s, dim, f);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256)
+k_gather(const c64* __restrict__ s, uint64_t dim, const uint64_t* __restrict__ idx, uint64_t cnt, c64* __restrict__ out)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < cnt) out[i] = idx[i] < dim ? s[idx[i]] : c64{0.0, 0.0};
+}
+
+hipError_t launch_gather(hipStream_t st, const c64* s, uint64_t dim, const uint64_t* idx, uint64_t cnt, c64* out)
+{
+    if (cnt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0, st, s, dim, idx, cnt, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_basis(hipStream_t st, c64* s, uint64_t dim)
 {
     hipLaunchKernelGGL(k_fill_basis, dim3(grid_for(dim, 256)), dim3(256), 0, st, s, dim);

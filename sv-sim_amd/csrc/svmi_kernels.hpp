@@ -24,6 +24,10 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
                         const uint64_t* hitab, double* grad_partials);
 hipError_t launch_standalone(hipStream_t st, c64* s, int n, const StandaloneOp& so, const c64* coef);
 hipError_t launch_fill_basis(hipStream_t st, c64* s, uint64_t dim);
+hipError_t launch_gather(hipStream_t st, const c64* s, uint64_t dim, const uint64_t* idx, uint64_t cnt, c64* out);
+uint32_t random_state_grid(uint64_t dim);
+hipError_t launch_random_state(hipStream_t st, c64* s, uint64_t dim, uint64_t seed, double* partials);
+hipError_t launch_scale(hipStream_t st, c64* s, uint64_t dim, double f);
 hipError_t launch_copy(hipStream_t st, c64* d, const c64* s, uint64_t dim);
 hipError_t launch_block_totals(hipStream_t st, const c64* s, uint64_t nblocks, uint32_t nthr,
                                uint32_t chunk, double* totals);

@@ -95,9 +95,66 @@ bool is_perm(const LOp& op, const int* src /* out row r <- in col src[r] */)
     return true;
 }
 
+// diag whose entries are all 1 except the all-ones index
+bool is_cphase(const SOp& s)
+{
+    const LOp& op = *s.op;
+    if (op.kind != LOp::DIAG) return false;
+    const size_t D = size_t(1) << op.k;
+    for (size_t r = 0; r + 1 < D; r++)
+        if (op.m[r].re != 1.0 || op.m[r].im != 0.0) return false;
+    return true;
+}
+
 struct Encoder {
     Program& prog;
     explicit Encoder(Program& p) : prog(p) {}
+
+    // ops [i, j) are controlled phases; emit one DOP_PHASESET per distinct
+    // register condition
+    void encode_phasesets(const std::vector<const SOp*>& ops, size_t i, size_t j, const int* tloc,
+                          const int* slot_of)
+    {
+        std::vector<std::pair<uint32_t, std::vector<PhaseEntry>>> groups;
+        for (size_t t = i; t < j; t++) {
+            const LOp& op = *ops[t]->op;
+            uint32_t rbits = 0;
+            PhaseEntry e;
+            std::memset(&e, 0, sizeof(e));
+            for (int q = 0; q < op.k; q++) {
+                const int l = tloc[op.q[q]];
+                if (l < 0) e.emask |= uint64_t(1) << op.q[q];
+                else if (slot_of[l] >= 0) rbits |= 1u << slot_of[l];
+                else e.gmask |= 1u << l;
+            }
+            const c64 ph = op.m[(size_t(1) << op.k) - 1];
+            e.ph_re = ph.re;
+            e.ph_im = ph.im;
+            uint32_t rsel = 0;
+            for (int r = 0; r < kRegs; r++)
+                if ((uint32_t(r) & rbits) == rbits) rsel |= 1u << r;
+            size_t g = 0;
+            while (g < groups.size() && groups[g].first != rsel) g++;
+            if (g == groups.size()) groups.push_back({rsel, {}});
+            groups[g].second.push_back(e);
+        }
+        for (auto& gr : groups) {
+            DOp d;
+            std::memset(&d, 0, sizeof(d));
+            d.targets = prog.n_states == 2 ? DOP_TO_BOTH : DOP_TO_PSI;
+            d.h.type = DOP_PHASESET;
+            d.h.rsel = gr.first;
+            d.h.s0 = uint32_t(gr.second.size());
+            d.h.coef = uint32_t(prog.coef.size());
+            for (const PhaseEntry& e : gr.second) {
+                c64 two[2];
+                std::memcpy(two, &e, sizeof(e));
+                prog.coef.push_back(two[0]);
+                prog.coef.push_back(two[1]);
+            }
+            prog.ops.push_back(d);
+        }
+    }
 
     uint32_t push_coef(const std::vector<c64>& v)
     {
@@ -431,13 +488,13 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             const uint32_t lm = local_mask(s);
             if (s->diag) {
                 if (ph) ph->push_back(s);
-                if (all_local(s) && (lm & ~R) == 0) score += 6;          // merges into a run
-                else if (lm & R) score -= 6;                             // splits a run
+                if (all_local(s) && (lm & ~R) == 0) score += 1;          // merges into a run
+                else if (lm & R) score -= 1;                             // splits a run
                 continue;
             }
             if ((lm & ~R) == 0) {
                 if (ph) ph->push_back(s);
-                score += reg_cost(*s->op);
+                score += 8 * reg_cost(*s->op);                           // progress first
             } else {
                 blocked |= s->dep;
                 if (rest) rest->push_back(s);
@@ -495,7 +552,17 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             rm.flush();
         }
         pi.op_begin = uint32_t(prog.ops.size());
-        for (const SOp* s : merged) {
+        for (size_t i = 0; i < merged.size();) {
+            // a run of controlled-phase ops commutes; group it by register
+            // condition into phase-product ops
+            size_t j = i;
+            while (j < merged.size() && is_cphase(*merged[j]) && !merged[j]->grad) j++;
+            if (j - i >= 2) {
+                enc.encode_phasesets(merged, i, j, tloc, slot_of);
+                i = j;
+                continue;
+            }
+            const SOp* s = merged[i++];
             uint32_t gslot = 0;
             if (s->grad) {
                 gslot = n_grad_sweep++;
@@ -660,8 +727,9 @@ std::string describe_program(const Program& prog)
                 ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3] << " :";
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                     const DOp& d = prog.ops[o];
-                    static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG"};
+                    static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS"};
                     ss << " " << nm[d.h.type];
+                    if (d.h.type == DOP_PHASESET) ss << d.h.s0;
                     if (d.h.type == DOP_DENSE || d.h.type == DOP_GRAD_DENSE) ss << __builtin_popcount(d.h.smask);
                 }
                 ss << "\n";

@@ -35,8 +35,8 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
 OPT_FUSION, OPT_TILE_QUBITS, OPT_LOW_QUBITS, OPT_SWEEP_FLOPS, OPT_PROFILE = 0, 1, 2, 3, 4
 
 # Every exported entry point (include/svmi.h, include/svmi_debug.h).
-EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
-           "svmi_set_state", "svmi_get_state", "svmi_upload", "svmi_clear_circuit",
+EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state", "svmi_set_random_state",
+           "svmi_set_state", "svmi_get_state", "svmi_get_amplitudes", "svmi_upload", "svmi_clear_circuit",
            "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_plan"]
@@ -81,7 +81,9 @@ def lib():
         L.svmi_num_qubits.restype = ctypes.c_int
         L.svmi_reset_state.argtypes = [_vp]
         L.svmi_set_state.argtypes = [_vp, _vp, _vp]
+        L.svmi_set_random_state.argtypes = [_vp, ctypes.c_uint64]
         L.svmi_get_state.argtypes = [_vp, _vp, _vp]
+        L.svmi_get_amplitudes.argtypes = [_vp, _vp, _sz, _vp, _vp]
         L.svmi_upload.argtypes = [_vp, _vp, _sz]
         L.svmi_clear_circuit.argtypes = [_vp]
         L.svmi_run.argtypes = [_vp]
@@ -169,6 +171,10 @@ class Engine:
     def reset_state(self):
         _check(lib().svmi_reset_state(self._h))
 
+    def set_random_state(self, seed):
+        """Seeded random unit state generated on the device (see random_amplitudes)."""
+        _check(lib().svmi_set_random_state(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF))
+
     def set_state(self, re, im):
         re = np.ascontiguousarray(re, dtype=np.float64)
         im = np.ascontiguousarray(im, dtype=np.float64)
@@ -180,6 +186,13 @@ class Engine:
         re = np.empty(1 << self.n)
         im = np.empty(1 << self.n)
         _check(lib().svmi_get_state(self._h, _ptr(re), _ptr(im)))
+        return re, im
+
+    def get_amplitudes(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        re = np.empty(len(idx))
+        im = np.empty(len(idx))
+        _check(lib().svmi_get_amplitudes(self._h, _ptr(idx), len(idx), _ptr(re), _ptr(im)))
         return re, im
 
     def upload(self, gates):
@@ -224,6 +237,27 @@ class Engine:
         s = Stats()
         _check(lib().svmi_get_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+
+def _splitmix64(z):
+    z = (z + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(0xFFFFFFFFFFFFFFFF)
+    return z ^ (z >> np.uint64(31))
+
+
+def random_amplitudes(seed, idx):
+    """Unnormalised amplitudes of Engine.set_random_state(seed) at indices idx
+    (the same counter-based generator, evaluated on the host)."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        s = np.uint64(int(seed) & 0xFFFFFFFFFFFFFFFF)
+        u0 = ((_splitmix64(s + np.uint64(2) * idx + np.uint64(1)) >> np.uint64(11)) + np.uint64(1)).astype(np.float64)
+        u1 = ((_splitmix64(s + np.uint64(2) * idx + np.uint64(2)) >> np.uint64(11)) + np.uint64(1)).astype(np.float64)
+    u0 *= 2.0 ** -53
+    u1 *= 2.0 ** -53
+    r = np.sqrt(-2.0 * np.log(u0))
+    return r * np.cos(2 * np.pi * u1), r * np.sin(2 * np.pi * u1)
 
 
 def batch_adjoint(items, device=0):

@@ -123,6 +123,61 @@ def test_qft_closed_form_n24():
     assert np.abs(got[0] + 1j * got[1] - want).max() < TOL
 
 
+def test_random_state_generator_matches_host():
+    n = 14
+    with svmi.Engine(n) as e:
+        e.set_random_state(1234)
+        re, im = e.get_state()
+        sub = e.get_amplitudes([0, 5, (1 << n) - 1])
+    hr, hi = svmi.random_amplitudes(1234, np.arange(1 << n))
+    nrm = np.sqrt(np.sum(hr * hr + hi * hi))
+    assert np.abs(re - hr / nrm).max() < 1e-12 and np.abs(im - hi / nrm).max() < 1e-12
+    assert abs(np.sum(re * re + im * im) - 1.0) < 1e-12
+    assert np.array_equal(sub[0], re[[0, 5, (1 << n) - 1]])
+
+
+def _inverse_qft(n):
+    g = W.qft(n=n)
+    inv = []
+    for row in g[::-1]:
+        name = svmi.OP_NAMES[int(row["op"])]
+        qs = [int(q) for q in row["qb"][: svmi.ARITY[name]]]
+        inv.append((int(row["op"]), qs, 0.0, 0.0, -row["lambda"]))
+    return inv
+
+
+def test_qft_n33_closed_form_and_round_trip():
+    # 33 qubits, 128 GiB state: QFT of a basis state has the closed form
+    # exp(2 pi i rev(x) rev(y) / N) / sqrt(N) (qubit 0 read as the most
+    # significant bit); QFT followed by its inverse returns a random input
+    n = 33
+    N = 1 << n
+    rng = np.random.default_rng(33)
+    ys = np.unique(rng.integers(0, N, size=2048, dtype=np.uint64))
+    x = 0b101100111000011110000011111000001
+
+    def rev(v):
+        return int(format(int(v), f"0{n}b")[::-1], 2)
+
+    with svmi.Engine(n) as e:
+        e.upload([(svmi.OP["X"], [q]) for q in range(n) if (x >> q) & 1])
+        e.run()
+        e.upload(W.qft(n=n))
+        e.run()
+        re, im = e.get_amplitudes(ys)
+        rx = rev(x)
+        ang = np.array([2 * np.pi * ((rx * rev(y)) % N) / N for y in ys])
+        want = np.exp(1j * ang) / np.sqrt(N)
+        assert np.abs(re + 1j * im - want).max() < 1e-13
+        e.set_random_state(2033)
+        r0, i0 = e.get_amplitudes(ys)
+        e.run()
+        e.upload(_inverse_qft(n))
+        e.run()
+        r1, i1 = e.get_amplitudes(ys)
+    assert max(np.abs(r1 - r0).max(), np.abs(i1 - i0).max()) < TOL
+
+
 def test_large_state_round_trip_n28():
     # n = 28 (4 GiB): circuit followed by its inverse returns the input state
     n = 28
