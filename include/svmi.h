@@ -156,7 +156,9 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
                          double* expval_out, double* grad_out);
 
 /* ---- batched expectation + adjoint gradient of many independent circuits,
- *      each started from |0..0>, in one persistent device launch. */
+ *      each started from |0..0>.  Step l of every circuit (forward sweeps,
+ *      <H>, adjoint sweeps, reduction) is dependency level l; one device
+ *      launch per level runs the tasks of all circuits at that level. */
 typedef struct svmi_batch_item {
     int32_t n_qubits;
     int32_t reserved;

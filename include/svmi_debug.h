@@ -32,6 +32,11 @@ svmi_status svmi_debug_emulate_adjoint(int n, const svmi_gate* gates, size_t n_g
                                        const double* re0, const double* im0,
                                        double* expval_out, double* grad_out);
 
+/* Host execution of svmi_batch_adjoint's task queue, level by level, the
+ * tasks of each level in reverse order (circuits of up to 20 qubits). */
+svmi_status svmi_debug_emulate_batch(const svmi_batch_item* items, size_t n_items,
+                                     double* expvals, double* grads);
+
 /* Planner summary: number of full-state passes and a text description
  * (truncated to buflen-1 bytes; buf may be NULL). */
 svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates,

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -207,6 +209,253 @@ void adjoint_entries(const svmi_gate* gates, size_t n_gates, const int32_t* para
     }
 }
 
+// ---- batch planning (shared by the device path and its host check) ----
+struct BatchPlan {
+    std::vector<Program> fwd, adj;
+    std::vector<SweepInfo> sweeps;
+    std::vector<PhaseInfo> phases;
+    std::vector<DOp> ops;
+    std::vector<c64> coef;
+    std::vector<uint64_t> hitab;
+    std::vector<PauliGroup> groups;
+    std::vector<PauliTerm> terms;
+    std::vector<BatchCircuit> circs;     // phi/lam unset; state_off gives the layout
+    std::vector<uint64_t> state_off;
+    uint64_t state_bytes = 0;
+    std::vector<BatchStep> steps;        // level by level
+    std::vector<uint32_t> level_begin;   // first step of each level, then steps.size()
+    uint64_t n_tasks = 0, n_e_partials = 0, n_grad_partials = 0;
+    uint32_t n_slots = 0;
+    std::vector<size_t> need_fill;       // circuits without forward sweeps
+};
+
+// Consistency of a batch plan: every index the kernel will follow is in range
+// and every dependency points to a step of the same circuit in an earlier
+// level (the level launches run in stream order).
+svmi_status verify_batch(const BatchPlan& bp)
+{
+    if (bp.level_begin.empty() || bp.level_begin.front() != 0 || bp.level_begin.back() != bp.steps.size())
+        return fail(SVMI_ERR_STATE, "batch plan: bad levels");
+    std::vector<uint32_t> level_of(bp.steps.size());
+    for (size_t l = 0; l + 1 < bp.level_begin.size(); l++) {
+        if (bp.level_begin[l] >= bp.level_begin[l + 1]) return fail(SVMI_ERR_STATE, "batch plan: empty level");
+        for (uint32_t s = bp.level_begin[l]; s < bp.level_begin[l + 1]; s++) level_of[s] = uint32_t(l);
+    }
+    uint64_t t = 0;
+    for (size_t s = 0; s < bp.steps.size(); s++) {
+        const BatchStep& b = bp.steps[s];
+        if (b.task_begin != t || b.n_tasks == 0 || b.circuit >= bp.circs.size())
+            return fail(SVMI_ERR_STATE, "batch plan: bad task range");
+        t += b.n_tasks;
+        if (b.dep >= 0 && (size_t(b.dep) >= s || bp.steps[b.dep].circuit != b.circuit ||
+                           level_of[size_t(b.dep)] >= level_of[s]))
+            return fail(SVMI_ERR_STATE, "batch plan: bad dependency");
+        if (b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ) {
+            if (b.sweep >= bp.sweeps.size()) return fail(SVMI_ERR_STATE, "batch plan: bad sweep index");
+            const SweepInfo& sw = bp.sweeps[b.sweep];
+            if (sw.n_tiles != b.n_tasks || sw.k > uint32_t(b.kind == BSTEP_ADJ ? kMaxTileQubitsAdj : kMaxTileQubits) ||
+                sw.k < uint32_t(kRegBits) || sw.phase_end > bp.phases.size() || sw.phase_begin >= sw.phase_end ||
+                sw.hitab + (1ull << (sw.k - sw.c)) > bp.hitab.size() || sw.n_grad > uint32_t(kMaxGradsPerSweep))
+                return fail(SVMI_ERR_STATE, "batch plan: bad sweep");
+            if ((sw.k + uint32_t(__builtin_popcountll(sw.extmask))) != bp.circs[b.circuit].n)
+                return fail(SVMI_ERR_STATE, "batch plan: sweep does not match the circuit size");
+            if (b.kind == BSTEP_ADJ && b.partial + sw.n_tiles * sw.n_grad > bp.n_grad_partials)
+                return fail(SVMI_ERR_STATE, "batch plan: bad gradient partial range");
+            for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
+                const PhaseInfo& ph = bp.phases[p];
+                if (ph.op_end > bp.ops.size() || ph.op_begin > ph.op_end)
+                    return fail(SVMI_ERR_STATE, "batch plan: bad phase");
+                for (int r = 0; r < kRegBits; r++)
+                    if (ph.r[r] >= sw.k) return fail(SVMI_ERR_STATE, "batch plan: bad register bit");
+                for (uint32_t o = ph.op_begin; o < ph.op_end; o++)
+                    if (bp.ops[o].h.coef > bp.coef.size()) return fail(SVMI_ERR_STATE, "batch plan: bad coef offset");
+            }
+        } else if (b.kind == BSTEP_PAULI) {
+            if (b.partial + b.n_tasks > bp.n_e_partials) return fail(SVMI_ERR_STATE, "batch plan: bad E partials");
+        }
+    }
+    if (t != bp.n_tasks) return fail(SVMI_ERR_STATE, "batch plan: task count");
+    for (const BatchCircuit& c : bp.circs)
+        if (c.group_end > bp.groups.size() || c.slot_begin + c.n_slots > bp.n_slots ||
+            c.e_partial + c.n_e_partials > bp.n_e_partials)
+            return fail(SVMI_ERR_STATE, "batch plan: bad circuit record");
+    return SVMI_OK;
+}
+
+svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const double* expvals, const double* grads,
+                        BatchPlan& bp)
+{
+    if (!items || !expvals) return fail(SVMI_ERR_ARG, "null batch arrays");
+    size_t total_params = 0;
+    for (size_t i = 0; i < n_items; i++) {
+        const svmi_batch_item& it = items[i];
+        if (it.n_qubits < 1 || it.n_qubits > 30)
+            return fail(SVMI_ERR_ARG, "batch item " + std::to_string(i) + ": n_qubits must be in [1, 30]");
+        SVMI_TRY(check_gates(it.gates, it.n_gates, it.n_qubits));
+        SVMI_TRY(check_terms(it.terms, it.n_terms, it.n_qubits));
+        SVMI_TRY(check_params(it.gates, it.n_gates, it.param_slots, it.n_params));
+        total_params += it.n_params;
+    }
+    if (total_params && !grads) return fail(SVMI_ERR_ARG, "null gradient array");
+    std::vector<int> ni(n_items);
+    std::vector<std::vector<PauliGroup>> pg(n_items);
+    std::vector<std::vector<PauliTerm>> pt(n_items);
+    bp.fwd.resize(n_items);
+    bp.adj.resize(n_items);
+    {
+        const unsigned nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<size_t> next{0};
+        auto worker = [&]() {
+            for (size_t i = next++; i < n_items; i = next++) {
+                const svmi_batch_item& it = items[i];
+                ni[i] = std::max(int(it.n_qubits), kRegBits);
+                PlanOptions of;
+                plan_forward(it.gates, it.n_gates, ni[i], of, bp.fwd[i]);
+                std::vector<AdjEntry> entries;
+                adjoint_entries(it.gates, it.n_gates, it.param_slots, entries);
+                PlanOptions oa;
+                oa.tile_qubits = kMaxTileQubitsAdj;
+                build_adjoint_program(entries, ni[i], oa, bp.adj[i]);
+                group_terms(it.terms, it.n_terms, pg[i], pt[i]);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t + 1 < nthr; t++) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
+    }
+    for (size_t i = 0; i < n_items; i++)
+        for (const Program* pr : {&bp.fwd[i], &bp.adj[i]})
+            for (const Step& stp : pr->steps)
+                if (stp.kind != Step::SWEEP)
+                    return fail(SVMI_ERR_UNSUPPORTED, "batch item " + std::to_string(i) +
+                                                          ": 5-qubit gates (C4X) are not supported in batches");
+    auto append_program = [&](const Program& pr) {
+        const uint32_t op_base = uint32_t(bp.ops.size()), coef_base = uint32_t(bp.coef.size());
+        const uint32_t ph_base = uint32_t(bp.phases.size()), ht_base = uint32_t(bp.hitab.size());
+        const uint32_t sw_base = uint32_t(bp.sweeps.size());
+        for (DOp d : pr.ops) {
+            d.h.coef += coef_base;
+            bp.ops.push_back(d);
+        }
+        for (PhaseInfo ph : pr.phases) {
+            ph.op_begin += op_base;
+            ph.op_end += op_base;
+            bp.phases.push_back(ph);
+        }
+        for (SweepInfo sw : pr.sweeps) {
+            sw.phase_begin += ph_base;
+            sw.phase_end += ph_base;
+            sw.hitab += ht_base;
+            bp.sweeps.push_back(sw);
+        }
+        bp.coef.insert(bp.coef.end(), pr.coef.begin(), pr.coef.end());
+        bp.hitab.insert(bp.hitab.end(), pr.hitab.begin(), pr.hitab.end());
+        return sw_base;
+    };
+    bp.circs.resize(n_items);
+    bp.state_off.resize(n_items);
+    std::vector<std::vector<BatchStep>> csteps(n_items);
+    for (size_t i = 0; i < n_items; i++) {
+        BatchCircuit& c = bp.circs[i];
+        std::memset(&c, 0, sizeof(c));
+        c.n = uint32_t(ni[i]);
+        const uint64_t dim = 1ull << ni[i];
+        bp.state_off[i] = bp.state_bytes;
+        bp.state_bytes += ((2 * dim * sizeof(c64)) + 255) & ~uint64_t(255);
+        c.group_begin = uint32_t(bp.groups.size());
+        for (PauliGroup g : pg[i]) {
+            g.begin += uint32_t(bp.terms.size());
+            g.end += uint32_t(bp.terms.size());
+            bp.groups.push_back(g);
+        }
+        bp.terms.insert(bp.terms.end(), pt[i].begin(), pt[i].end());
+        c.group_end = uint32_t(bp.groups.size());
+        auto& cs = csteps[i];
+        const Program& fw = bp.fwd[i];
+        const uint32_t fsw = append_program(fw);
+        for (size_t s = 0; s < fw.sweeps.size(); s++) {
+            BatchStep b;
+            std::memset(&b, 0, sizeof(b));
+            b.kind = BSTEP_FWD;
+            b.circuit = uint32_t(i);
+            b.sweep = fsw + uint32_t(s);
+            b.n_tasks = fw.sweeps[s].n_tiles;
+            b.init_zero = s == 0 ? 1u : 0u;
+            cs.push_back(b);
+        }
+        if (fw.sweeps.empty()) bp.need_fill.push_back(i);
+        BatchStep pb;
+        std::memset(&pb, 0, sizeof(pb));
+        pb.kind = BSTEP_PAULI;
+        pb.circuit = uint32_t(i);
+        pb.n_tasks = (dim + kPauliChunk - 1) / kPauliChunk;
+        pb.partial = bp.n_e_partials;
+        c.e_partial = uint32_t(bp.n_e_partials);
+        c.n_e_partials = uint32_t(pb.n_tasks);
+        bp.n_e_partials += pb.n_tasks;
+        cs.push_back(pb);
+        const Program& ad = bp.adj[i];
+        const uint32_t asw = append_program(ad);
+        c.slot_begin = bp.n_slots;
+        c.n_slots = ad.n_grad;
+        bp.n_slots += ad.n_grad;
+        for (size_t s = 0; s < ad.sweeps.size(); s++) {
+            BatchStep b;
+            std::memset(&b, 0, sizeof(b));
+            b.kind = BSTEP_ADJ;
+            b.circuit = uint32_t(i);
+            b.sweep = asw + uint32_t(s);
+            b.n_tasks = ad.sweeps[s].n_tiles;
+            b.partial = bp.n_grad_partials;
+            bp.n_grad_partials += ad.sweeps[s].n_tiles * ad.sweeps[s].n_grad;
+            cs.push_back(b);
+        }
+        BatchStep rb;
+        std::memset(&rb, 0, sizeof(rb));
+        rb.kind = BSTEP_REDUCE;
+        rb.circuit = uint32_t(i);
+        rb.n_tasks = 1;
+        cs.push_back(rb);
+    }
+    // level l = step l of every circuit that has one, largest circuits first
+    std::vector<size_t> corder(n_items);
+    std::iota(corder.begin(), corder.end(), 0);
+    std::stable_sort(corder.begin(), corder.end(), [&](size_t a, size_t b) {
+        return (csteps[a].size() << ni[a]) > (csteps[b].size() << ni[b]);
+    });
+    std::vector<std::vector<uint32_t>> idx_of(n_items);
+    size_t max_levels = 0;
+    for (auto& cs : csteps) max_levels = std::max(max_levels, cs.size());
+    for (size_t lv = 0; lv < max_levels; lv++) {
+        bp.level_begin.push_back(uint32_t(bp.steps.size()));
+        for (size_t i : corder) {
+            if (lv >= csteps[i].size()) continue;
+            BatchStep b = csteps[i][lv];
+            b.dep = lv == 0 ? -1 : int32_t(idx_of[i][lv - 1]);
+            b.task_begin = bp.n_tasks;
+            bp.n_tasks += b.n_tasks;
+            idx_of[i].push_back(uint32_t(bp.steps.size()));
+            bp.steps.push_back(b);
+        }
+    }
+    bp.level_begin.push_back(uint32_t(bp.steps.size()));
+    return verify_batch(bp);
+}
+
+void finish_batch(const BatchPlan& bp, const svmi_batch_item* items, size_t n_items, const std::vector<double>& gs,
+                  const std::vector<double>& eo, double* expvals, double* grads)
+{
+    size_t goff = 0;
+    for (size_t i = 0; i < n_items; i++) {
+        expvals[i] = eo[i];
+        for (size_t p = 0; p < items[i].n_params; p++) grads[goff + p] = 0.0;
+        for (uint32_t s = 0; s < bp.adj[i].n_grad; s++)
+            grads[goff + bp.adj[i].grad_param[s]] += 2.0 * gs[bp.circs[i].slot_begin + s];
+        goff += items[i].n_params;
+    }
+}
+
 }  // namespace
 
 struct svmi_sim {
@@ -230,6 +479,17 @@ struct svmi_sim {
 };
 
 namespace {
+
+svmi_status ensure_kernels()
+{
+    // per-process: kernel attributes (dynamic LDS limit) for the current device
+    static bool ready = false;
+    if (!ready) {
+        SVMI_HIP(init_sweep_kernels());
+        ready = true;
+    }
+    return SVMI_OK;
+}
 
 svmi_status use_device(svmi_handle h)
 {
@@ -341,11 +601,7 @@ svmi_status svmi_create(int n_qubits, int device, svmi_handle* out)
         return fail(SVMI_ERR_DEVICE, "device " + std::to_string(device) + " not present (" + std::to_string(ndev) +
                                          " visible)");
     SVMI_HIP(hipSetDevice(device));
-    static bool kernels_ready = false;
-    if (!kernels_ready) {
-        SVMI_HIP(init_sweep_kernels());
-        kernels_ready = true;
-    }
+    SVMI_TRY(ensure_kernels());
     svmi_sim* h = new svmi_sim();
     h->n = n_qubits;
     h->n_int = std::max(n_qubits, kRegBits);
@@ -649,24 +905,109 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
     return st;
 }
 
+// Batched expectation + adjoint gradients: every circuit is planned on the
+// host (in parallel), the programs are concatenated into one device program
+// and one launch per dependency level runs the tasks of every circuit at
+// that level (no host synchronisation between the levels).
 svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t n_items, double* expvals,
                                double* grads)
 {
-    if (n_items && (!items || !expvals)) return fail(SVMI_ERR_ARG, "null batch arrays");
-    size_t goff = 0;
-    for (size_t i = 0; i < n_items; i++) {
-        const svmi_batch_item& it = items[i];
-        svmi_handle h = nullptr;
-        SVMI_TRY(svmi_create(it.n_qubits, device, &h));
-        svmi_status st = svmi_upload(h, it.gates, it.n_gates);
-        if (st == SVMI_OK)
-            st = svmi_adjoint(h, it.terms, it.n_terms, it.param_slots, it.n_params, &expvals[i],
-                              grads ? grads + goff : nullptr);
-        svmi_destroy(h);
-        if (st != SVMI_OK) return st;
-        goff += it.n_params;
+    if (n_items == 0) return SVMI_OK;
+    BatchPlan bp;
+    SVMI_TRY(build_batch(items, n_items, expvals, grads, bp));
+    int ndev = 0;
+    SVMI_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(SVMI_ERR_DEVICE, "device " + std::to_string(device) + " not present");
+    SVMI_HIP(hipSetDevice(device));
+    SVMI_TRY(ensure_kernels());
+
+    DevBuf dstate, dsteps, dcircs, dsweeps, dphases, dops, dcoef, dhitab, dgroups, dterms, de_part, dg_part, dgsum,
+        de_out;
+    auto release_all = [&]() {
+        for (DevBuf* b : {&dstate, &dsteps, &dcircs, &dsweeps, &dphases, &dops, &dcoef, &dhitab, &dgroups, &dterms,
+                          &de_part, &dg_part, &dgsum, &de_out})
+            b->release();
+    };
+    hipStream_t st = nullptr;
+    svmi_status rc = SVMI_OK;
+    auto put = [&](DevBuf& b, const void* src, size_t bytes) -> svmi_status {
+        SVMI_TRY(b.ensure(std::max<size_t>(bytes, 16)));
+        if (bytes) SVMI_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+        return SVMI_OK;
+    };
+    do {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            rc = fail(SVMI_ERR_DEVICE, "batch: stream creation failed");
+            break;
+        }
+        if ((rc = dstate.ensure(bp.state_bytes)) != SVMI_OK) break;
+        std::vector<BatchCircuit> circs = bp.circs;
+        for (size_t i = 0; i < circs.size(); i++) {
+            circs[i].phi = reinterpret_cast<c64*>(static_cast<char*>(dstate.p) + bp.state_off[i]);
+            circs[i].lam = circs[i].phi + (1ull << circs[i].n);
+        }
+        for (size_t i : bp.need_fill) {
+            if (launch_fill_basis(st, circs[i].phi, 1ull << circs[i].n) != hipSuccess) {
+                rc = fail(SVMI_ERR_DEVICE, "batch: fill failed");
+                break;
+            }
+        }
+        if (rc != SVMI_OK) break;
+        if ((rc = put(dsteps, bp.steps.data(), bp.steps.size() * sizeof(BatchStep))) != SVMI_OK) break;
+        if ((rc = put(dcircs, circs.data(), circs.size() * sizeof(BatchCircuit))) != SVMI_OK) break;
+        if ((rc = put(dsweeps, bp.sweeps.data(), bp.sweeps.size() * sizeof(SweepInfo))) != SVMI_OK) break;
+        if ((rc = put(dphases, bp.phases.data(), bp.phases.size() * sizeof(PhaseInfo))) != SVMI_OK) break;
+        if ((rc = put(dops, bp.ops.data(), bp.ops.size() * sizeof(DOp))) != SVMI_OK) break;
+        if ((rc = put(dcoef, bp.coef.data(), bp.coef.size() * sizeof(c64))) != SVMI_OK) break;
+        if ((rc = put(dhitab, bp.hitab.data(), bp.hitab.size() * sizeof(uint64_t))) != SVMI_OK) break;
+        if ((rc = put(dgroups, bp.groups.data(), bp.groups.size() * sizeof(PauliGroup))) != SVMI_OK) break;
+        if ((rc = put(dterms, bp.terms.data(), bp.terms.size() * sizeof(PauliTerm))) != SVMI_OK) break;
+        if ((rc = de_part.ensure(std::max<uint64_t>(bp.n_e_partials, 1) * sizeof(double))) != SVMI_OK) break;
+        if ((rc = dg_part.ensure(std::max<uint64_t>(bp.n_grad_partials, 1) * sizeof(double))) != SVMI_OK) break;
+        if ((rc = dgsum.ensure(std::max<uint32_t>(bp.n_slots, 1) * sizeof(double))) != SVMI_OK) break;
+        if ((rc = de_out.ensure(n_items * sizeof(double))) != SVMI_OK) break;
+        BatchArgs a;
+        a.steps = dsteps.as<BatchStep>();
+        a.circuits = dcircs.as<BatchCircuit>();
+        a.sweeps = dsweeps.as<SweepInfo>();
+        a.phases = dphases.as<PhaseInfo>();
+        a.ops = dops.as<DOp>();
+        a.coef = dcoef.as<c64>();
+        a.hitab = dhitab.as<uint64_t>();
+        a.groups = dgroups.as<PauliGroup>();
+        a.terms = dterms.as<PauliTerm>();
+        a.e_partials = de_part.as<double>();
+        a.grad_partials = dg_part.as<double>();
+        a.gsum = dgsum.as<double>();
+        a.e_out = de_out.as<double>();
+        hipError_t e = hipSuccess;
+        for (size_t l = 0; l + 1 < bp.level_begin.size() && e == hipSuccess; l++) {
+            const uint32_t s0 = bp.level_begin[l], s1 = bp.level_begin[l + 1];
+            const uint64_t t0 = bp.steps[s0].task_begin;
+            const uint64_t t1 = bp.steps[s1 - 1].task_begin + bp.steps[s1 - 1].n_tasks;
+            e = launch_batch_level(st, a, s0, s1, t0, t1 - t0);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            rc = fail(SVMI_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+            break;
+        }
+        std::vector<double> gs(bp.n_slots), eo(n_items);
+        if (bp.n_slots)
+            e = hipMemcpy(gs.data(), dgsum.p, bp.n_slots * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(eo.data(), de_out.p, n_items * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            rc = fail(SVMI_ERR_DEVICE, std::string("batch results: ") + hipGetErrorString(e));
+            break;
+        }
+        finish_batch(bp, items, n_items, gs, eo, expvals, grads);
+    } while (false);
+    if (st) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
     }
-    return SVMI_OK;
+    release_all();
+    return rc;
 }
 
 svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
@@ -758,6 +1099,54 @@ void host_standalone(std::vector<c64>& s, int n, const StandaloneOp& so, const s
     }
 }
 
+// One tile of one sweep on the host (the kernels' sweep_tile without LDS).
+// gsum_tile receives the tile's gradient accumulators (ns == 2).
+void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, const c64* coef, const uint64_t* hitab,
+               int ns, c64* s0, c64* s1, uint64_t tile, bool init_zero, double* gsum_tile)
+{
+    const uint32_t tsize = 1u << sw.k, ngroups = tsize >> kRegBits;
+    const uint32_t cmask = (1u << sw.c) - 1u;
+    std::vector<c64> t0(tsize), t1(tsize);
+    std::vector<uint64_t> gidx(tsize);
+    for (uint32_t i = 0; i < sw.n_grad; i++) gsum_tile[i] = 0.0;
+    const uint64_t base = tile_base(sw, tile);
+    for (uint32_t l = 0; l < tsize; l++) {
+        gidx[l] = base | hitab[sw.hitab + (l >> sw.c)] | (l & cmask);
+        t0[l] = init_zero ? c64{gidx[l] == 0 ? 1.0 : 0.0, 0.0} : s0[gidx[l]];
+        if (ns == 2) t1[l] = s1[gidx[l]];
+    }
+    for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
+        const PhaseInfo& ph = phases[p];
+        uint32_t off[kRegs];
+        phase_offsets(ph, off);
+        for (uint32_t g = 0; g < ngroups; g++) {
+            const uint32_t goff = group_offset(ph, g);
+            c64 a[kRegs], b[kRegs];
+            for (int r = 0; r < kRegs; r++) {
+                a[r] = t0[goff | off[r]];
+                b[r] = ns == 2 ? t1[goff | off[r]] : c64{0.0, 0.0};
+            }
+            for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
+                const DOp& op = ops[o];
+                if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG) {
+                    gsum_tile[op.h.grad] += exec_grad_op(op.h, &op, a, b, coef, goff, off, base);
+                } else {
+                    exec_state_op(op.h, &op, a, coef, goff, off, base);
+                    if (ns == 2) exec_state_op(op.h, &op, b, coef, goff, off, base);
+                }
+            }
+            for (int r = 0; r < kRegs; r++) {
+                t0[goff | off[r]] = a[r];
+                if (ns == 2) t1[goff | off[r]] = b[r];
+            }
+        }
+    }
+    for (uint32_t l = 0; l < tsize; l++) {
+        s0[gidx[l]] = t0[l];
+        if (ns == 2) s1[gidx[l]] = t1[l];
+    }
+}
+
 void host_program(const Program& prog, std::vector<c64>& s0, std::vector<c64>& s1, std::vector<double>& gsum)
 {
     const int ns = prog.n_states;
@@ -769,48 +1158,11 @@ void host_program(const Program& prog, std::vector<c64>& s0, std::vector<c64>& s
             continue;
         }
         const SweepInfo& sw = prog.sweeps[st.index];
-        const uint32_t tsize = 1u << sw.k, ngroups = tsize >> kRegBits;
-        const uint32_t cmask = (1u << sw.c) - 1u;
-        std::vector<c64> t0(tsize), t1(tsize);
-        std::vector<uint64_t> gidx(tsize);
+        std::vector<double> g(sw.n_grad);
         for (uint64_t tile = 0; tile < sw.n_tiles; tile++) {
-            const uint64_t base = tile_base(sw, tile);
-            for (uint32_t l = 0; l < tsize; l++) {
-                gidx[l] = base | prog.hitab[sw.hitab + (l >> sw.c)] | (l & cmask);
-                t0[l] = s0[gidx[l]];
-                if (ns == 2) t1[l] = s1[gidx[l]];
-            }
-            for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
-                const PhaseInfo& ph = prog.phases[p];
-                uint32_t off[kRegs];
-                phase_offsets(ph, off);
-                for (uint32_t g = 0; g < ngroups; g++) {
-                    const uint32_t goff = group_offset(ph, g);
-                    c64 a[kRegs], b[kRegs];
-                    for (int r = 0; r < kRegs; r++) {
-                        a[r] = t0[goff | off[r]];
-                        b[r] = ns == 2 ? t1[goff | off[r]] : c64{0.0, 0.0};
-                    }
-                    for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
-                        const DOp& op = prog.ops[o];
-                        if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG) {
-                            gsum[sw.grad_base + op.h.grad] +=
-                                exec_grad_op(op.h, &op, a, b, prog.coef.data(), goff, off, base);
-                        } else {
-                            exec_state_op(op.h, &op, a, prog.coef.data(), goff, off, base);
-                            if (ns == 2) exec_state_op(op.h, &op, b, prog.coef.data(), goff, off, base);
-                        }
-                    }
-                    for (int r = 0; r < kRegs; r++) {
-                        t0[goff | off[r]] = a[r];
-                        if (ns == 2) t1[goff | off[r]] = b[r];
-                    }
-                }
-            }
-            for (uint32_t l = 0; l < tsize; l++) {
-                s0[gidx[l]] = t0[l];
-                if (ns == 2) s1[gidx[l]] = t1[l];
-            }
+            host_tile(sw, prog.phases.data(), prog.ops.data(), prog.coef.data(), prog.hitab.data(), ns, s0.data(),
+                      ns == 2 ? s1.data() : nullptr, tile, false, g.data());
+            for (uint32_t i = 0; i < sw.n_grad; i++) gsum[sw.grad_base + i] += g[i];
         }
     }
 }
@@ -906,6 +1258,77 @@ svmi_status svmi_debug_emulate_adjoint(int n, const svmi_gate* gates, size_t n_g
     host_program(bp, psi, lam, g);
     for (size_t p = 0; p < n_params; p++) grad_out[p] = 0.0;
     for (uint32_t i = 0; i < bp.n_grad; i++) grad_out[bp.grad_param[i]] += 2.0 * g[i];
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_emulate_batch(const svmi_batch_item* items, size_t n_items, double* expvals, double* grads)
+{
+    if (n_items == 0) return SVMI_OK;
+    for (size_t i = 0; i < n_items; i++)
+        if (items && items[i].n_qubits > 20) return fail(SVMI_ERR_ARG, "batch emulation supports up to 20 qubits");
+    BatchPlan bp;
+    SVMI_TRY(build_batch(items, n_items, expvals, grads, bp));
+    std::vector<c64> mem(bp.state_bytes / sizeof(c64) + 1);
+    auto phi = [&](size_t c) { return mem.data() + bp.state_off[c] / sizeof(c64); };
+    for (size_t i : bp.need_fill) {
+        c64* p = phi(i);
+        for (uint64_t x = 0; x < (1ull << bp.circs[i].n); x++) p[x] = c64{x == 0 ? 1.0 : 0.0, 0.0};
+    }
+    std::vector<double> e_part(bp.n_e_partials), g_part(bp.n_grad_partials), gs(bp.n_slots), eo(n_items);
+    // level by level; inside a level the tasks run last to first, since the
+    // device runs them in no particular order
+    for (size_t l = 0; l + 1 < bp.level_begin.size(); l++)
+    for (uint32_t si = bp.level_begin[l + 1]; si-- > bp.level_begin[l];) {
+        const BatchStep& st = bp.steps[si];
+        const BatchCircuit& c = bp.circs[st.circuit];
+        c64* ph = phi(st.circuit);
+        c64* lm = ph + (1ull << c.n);
+        for (uint64_t task = st.n_tasks; task-- > 0;) {
+            if (st.kind == BSTEP_FWD || st.kind == BSTEP_ADJ) {
+                const SweepInfo& sw = bp.sweeps[st.sweep];
+                std::vector<double> g(sw.n_grad);
+                host_tile(sw, bp.phases.data(), bp.ops.data(), bp.coef.data(), bp.hitab.data(),
+                          st.kind == BSTEP_ADJ ? 2 : 1, ph, st.kind == BSTEP_ADJ ? lm : nullptr, task,
+                          st.init_zero != 0, g.data());
+                for (uint32_t i = 0; i < sw.n_grad; i++) g_part[st.partial + task * sw.n_grad + i] = g[i];
+            } else if (st.kind == BSTEP_PAULI) {
+                const uint64_t dim = 1ull << c.n;
+                const uint64_t y0 = task * kPauliChunk, y1 = std::min<uint64_t>(y0 + kPauliChunk, dim);
+                double acc = 0.0;
+                for (uint64_t y = y0; y < y1; y++) {
+                    c64 l{0.0, 0.0};
+                    for (uint32_t gi = c.group_begin; gi < c.group_end; gi++) {
+                        const PauliGroup& G = bp.groups[gi];
+                        const uint64_t x = y ^ G.xmask;
+                        double fr = 0.0, fi = 0.0;
+                        for (uint32_t t = G.begin; t < G.end; t++) {
+                            const bool neg = __builtin_popcountll(x & bp.terms[t].zmask) & 1;
+                            fr += neg ? -bp.terms[t].cre : bp.terms[t].cre;
+                            fi += neg ? -bp.terms[t].cim : bp.terms[t].cim;
+                        }
+                        l = cmac(l, c64{fr, fi}, ph[x]);
+                    }
+                    lm[y] = l;
+                    acc += ph[y].re * l.re + ph[y].im * l.im;
+                }
+                e_part[st.partial + task] = acc;
+            } else {
+                for (int32_t s2 = st.dep; s2 >= 0 && bp.steps[s2].kind == BSTEP_ADJ; s2 = bp.steps[s2].dep) {
+                    const BatchStep& as = bp.steps[s2];
+                    const SweepInfo& sw = bp.sweeps[as.sweep];
+                    for (uint32_t i = 0; i < sw.n_grad; i++) {
+                        double v = 0.0;
+                        for (uint64_t t = 0; t < sw.n_tiles; t++) v += g_part[as.partial + t * sw.n_grad + i];
+                        gs[c.slot_begin + sw.grad_base + i] = v;
+                    }
+                }
+                double e = 0.0;
+                for (uint32_t j = 0; j < c.n_e_partials; j++) e += e_part[c.e_partial + j];
+                eo[st.circuit] = e;
+            }
+        }
+    }
+    finish_batch(bp, items, n_items, gs, eo, expvals, grads);
     return SVMI_OK;
 }
 

@@ -19,14 +19,17 @@ namespace svmi {
 
 // ------------------------------------------------------------------ sweeps
 
+// One tile of one sweep.  Threads tid < nt (= 2^(k-4)) own one register
+// group per phase; the others (batched kernel: blockDim > nt) only join the
+// barriers.  init_zero: the tile is generated as |0..0> instead of loaded.
 template <int NS>
 __device__ __attribute__((always_inline)) inline void
-sweep_body(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
+sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
            const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
            const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
-           double* __restrict__ grad_partials, uint64_t tile_offset)
+           double* __restrict__ grad_partials, uint64_t tile, uint32_t nt, unsigned char* smem,
+           bool init_zero)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t k = sw.k;
     const uint32_t tsize = 1u << k;
     c64* t0 = reinterpret_cast<c64*>(smem);
@@ -34,70 +37,77 @@ sweep_body(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     double* gacc = reinterpret_cast<double*>(t0 + tsize * NS);
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t nt = blockDim.x;            // == 2^(k-4)
+    const bool active = tid < nt;
     const uint32_t nwaves = (nt + 63) / 64;
-    const uint64_t tile = tile_offset + blockIdx.x;
     const uint64_t base = tile_base(sw, tile);
     const uint32_t cmask = (1u << sw.c) - 1u;
     const uint64_t* ht = hitab + sw.hitab;
 
+    if (active) {
 #pragma unroll
-    for (int j = 0; j < kRegs; j++) {
-        const uint32_t l = tid + nt * j;
-        const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
-        t0[lds_slot(l)] = s0[g];
-        if (NS == 2) t1[lds_slot(l)] = s1[g];
+        for (int j = 0; j < kRegs; j++) {
+            const uint32_t l = tid + nt * j;
+            const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
+            t0[lds_slot(l)] = init_zero ? c64{g == 0 ? 1.0 : 0.0, 0.0} : s0[g];
+            if (NS == 2) t1[lds_slot(l)] = s1[g];
+        }
     }
     if (NS == 2)
-        for (uint32_t i = tid; i < sw.n_grad * nwaves; i += nt) gacc[i] = 0.0;
+        for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
     __syncthreads();
 
+    // every thread runs the phase loop (uniform barriers); only threads that
+    // own a register group do work
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
-        const PhaseInfo ph = phases[p];
-        uint32_t off[kRegs];
-        phase_offsets(ph, off);
-        const uint32_t goff = group_offset(ph, tid);
-        c64 a[kRegs];
-        c64 b[kRegs];
+        if (active) {
+            const PhaseInfo ph = phases[p];
+            uint32_t off[kRegs];
+            phase_offsets(ph, off);
+            const uint32_t goff = group_offset(ph, tid);
+            c64 a[kRegs];
+            c64 b[kRegs];
 #pragma unroll
-        for (int r = 0; r < kRegs; r++) {
-            a[r] = t0[lds_slot(goff | off[r])];
-            if (NS == 2) b[r] = t1[lds_slot(goff | off[r])];
-        }
-        // op headers are fetched one op ahead so the scalar-load latency of
-        // the next op overlaps the arithmetic of the current one
-        DOpHdr nxt;
-        if (ph.op_begin < ph.op_end) nxt = ops[ph.op_begin].h;
-        for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
-            const DOpHdr h = nxt;
-            if (o + 1 < ph.op_end) nxt = ops[o + 1].h;
-            if (NS == 2 && (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG)) {
-                double v = exec_grad_op(h, ops + o, a, b, coef, goff, off, base);
-                const int width = nt < 64 ? int(nt) : 64;
-                for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
-                if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
-            } else {
-                exec_state_op(h, ops + o, a, coef, goff, off, base);
-                if (NS == 2) exec_state_op(h, ops + o, b, coef, goff, off, base);
+            for (int r = 0; r < kRegs; r++) {
+                a[r] = t0[lds_slot(goff | off[r])];
+                if (NS == 2) b[r] = t1[lds_slot(goff | off[r])];
             }
-        }
+            // op headers are fetched one op ahead so the scalar-load latency
+            // of the next op overlaps the arithmetic of the current one
+            DOpHdr nxt;
+            if (ph.op_begin < ph.op_end) nxt = ops[ph.op_begin].h;
+            for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
+                const DOpHdr h = nxt;
+                if (o + 1 < ph.op_end) nxt = ops[o + 1].h;
+                if (NS == 2 && (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG)) {
+                    double v = exec_grad_op(h, ops + o, a, b, coef, goff, off, base);
+                    const int width = nt < 64 ? int(nt) : 64;
+                    for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
+                    if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
+                } else {
+                    exec_state_op(h, ops + o, a, coef, goff, off, base);
+                    if (NS == 2) exec_state_op(h, ops + o, b, coef, goff, off, base);
+                }
+            }
 #pragma unroll
-        for (int r = 0; r < kRegs; r++) {
-            t0[lds_slot(goff | off[r])] = a[r];
-            if (NS == 2) t1[lds_slot(goff | off[r])] = b[r];
+            for (int r = 0; r < kRegs; r++) {
+                t0[lds_slot(goff | off[r])] = a[r];
+                if (NS == 2) t1[lds_slot(goff | off[r])] = b[r];
+            }
         }
         __syncthreads();
     }
 
+    if (active) {
 #pragma unroll
-    for (int j = 0; j < kRegs; j++) {
-        const uint32_t l = tid + nt * j;
-        const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
-        s0[g] = t0[lds_slot(l)];
-        if (NS == 2) s1[g] = t1[lds_slot(l)];
+        for (int j = 0; j < kRegs; j++) {
+            const uint32_t l = tid + nt * j;
+            const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
+            s0[g] = t0[lds_slot(l)];
+            if (NS == 2) s1[g] = t1[lds_slot(l)];
+        }
     }
     if (NS == 2) {
-        for (uint32_t i = tid; i < sw.n_grad; i += nt) {
+        for (uint32_t i = tid; i < sw.n_grad; i += blockDim.x) {
             double v = 0.0;
             for (uint32_t w = 0; w < nwaves; w++) v += gacc[i * nwaves + w];
             grad_partials[tile * sw.n_grad + i] = v;
@@ -105,15 +115,15 @@ sweep_body(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     }
 }
 
-static size_t g_max_smem = 64 * 1024;  // dynamic LDS the sweep kernels may use
-
 // One state: 64 KiB tile -> 2 workgroups (8 waves) per CU, so 256 VGPRs.
 __global__ void __launch_bounds__(kThreads, 2)
 k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
          uint64_t tile_offset)
 {
-    sweep_body<1>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset);
+    extern __shared__ __align__(16) unsigned char smem[];
+    sweep_tile<1>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x, smem,
+                  false);
 }
 
 // Two states (adjoint): one wave per SIMD, registers for both groups.
@@ -122,7 +132,129 @@ k_sweep2(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw, const P
          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
          double* __restrict__ grad_partials, uint64_t tile_offset)
 {
-    sweep_body<2>(s0, s1, sw, phases, ops, coef, hitab, grad_partials, tile_offset);
+    extern __shared__ __align__(16) unsigned char smem[];
+    sweep_tile<2>(s0, s1, sw, phases, ops, coef, hitab, grad_partials, tile_offset + blockIdx.x, blockDim.x, smem,
+                  false);
+}
+
+static size_t g_max_smem = 64 * 1024;  // dynamic LDS the sweep kernels may use
+static bool g_batch_ok = false;         // batch kernel's LDS attribute was accepted
+
+// --------------------------------------------------------------- batching
+//
+// A batch is a queue of steps (forward sweep, Pauli step, adjoint sweep,
+// reduction) of many circuits; a task is one tile (or chunk) of one step.
+// Steps are grouped into dependency levels: level l holds step l of every
+// circuit that has one, so the tasks of one level are independent of each
+// other.  One launch covers one level, blockIdx.x -> task; stream order
+// between the level launches supplies every dependency, so no workgroup
+// ever waits for another one.
+
+constexpr uint32_t kBatchThreads = 256;
+constexpr size_t kBatchSmem = size_t(16) << kMaxTileQubits;   // 64 KiB: one 2^12 tile or two 2^11 tiles
+constexpr size_t kBatchGradLds = size_t(8) * kMaxGradsPerSweep * 4;
+
+size_t batch_smem_bytes() { return kBatchSmem + kBatchGradLds; }
+
+__device__ void batch_pauli(const BatchArgs& a, const BatchStep& st, const BatchCircuit& c, uint64_t task,
+                            double* red)
+{
+    const uint64_t dim = 1ull << c.n;
+    const uint64_t y0 = task * kPauliChunk;
+    const uint64_t y1 = y0 + kPauliChunk < dim ? y0 + kPauliChunk : dim;
+    double acc = 0.0;
+    for (uint64_t y = y0 + threadIdx.x; y < y1; y += blockDim.x) {
+        c64 l{0.0, 0.0};
+        for (uint32_t g = c.group_begin; g < c.group_end; g++) {
+            const PauliGroup G = a.groups[g];
+            const uint64_t x = y ^ G.xmask;
+            double fr = 0.0, fi = 0.0;
+            for (uint32_t t = G.begin; t < G.end; t++) {
+                const PauliTerm T = a.terms[t];
+                const bool neg = __popcll(x & T.zmask) & 1;
+                fr += neg ? -T.cre : T.cre;
+                fi += neg ? -T.cim : T.cim;
+            }
+            l = cmac(l, c64{fr, fi}, c.phi[x]);
+        }
+        c.lam[y] = l;
+        const c64 p = c.phi[y];
+        acc += p.re * l.re + p.im * l.im;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.e_partials[st.partial + task] = red[0];
+}
+
+__device__ void batch_reduce(const BatchArgs& a, const BatchCircuit& c, uint32_t ci, int32_t last)
+{
+    // gradient accumulators of the circuit's adjoint sweeps (found by walking
+    // its dependency chain back from the last one), each summed over its
+    // tiles in tile order; then <H> from the Pauli partials
+    for (int32_t s = last; s >= 0 && a.steps[s].kind == BSTEP_ADJ; s = a.steps[s].dep) {
+        const BatchStep st = a.steps[s];
+        const SweepInfo& sw = a.sweeps[st.sweep];
+        for (uint32_t i = threadIdx.x; i < sw.n_grad; i += blockDim.x) {
+            double v = 0.0;
+            for (uint64_t t = 0; t < sw.n_tiles; t++) v += a.grad_partials[st.partial + t * sw.n_grad + i];
+            a.gsum[c.slot_begin + sw.grad_base + i] = v;
+        }
+    }
+    if (threadIdx.x == 0) {
+        double e = 0.0;
+        for (uint32_t j = 0; j < c.n_e_partials; j++) e += a.e_partials[c.e_partial + j];
+        a.e_out[ci] = e;
+    }
+}
+
+// One task of the level made of steps [step_begin, step_end): the task
+// number is task0 + blockIdx.x, its step the last one whose task_begin is
+// not above it.
+__global__ void __launch_bounds__(kBatchThreads, 1)
+k_batch(BatchArgs a, uint32_t step_begin, uint32_t step_end, uint64_t task0)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[kBatchThreads];
+    const uint64_t t = task0 + blockIdx.x;
+    uint32_t lo = step_begin, hi = step_end;
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (a.steps[mid].task_begin <= t) lo = mid;
+        else hi = mid;
+    }
+    const BatchStep st = a.steps[lo];
+    const uint64_t task = t - st.task_begin;
+    const BatchCircuit& c = a.circuits[st.circuit];
+    if (st.kind == BSTEP_FWD) {
+        const SweepInfo& sw = a.sweeps[st.sweep];
+        sweep_tile<1>(c.phi, nullptr, sw, a.phases, a.ops, a.coef, a.hitab, nullptr, task,
+                      1u << (sw.k - kRegBits), smem, st.init_zero != 0);
+    } else if (st.kind == BSTEP_ADJ) {
+        const SweepInfo& sw = a.sweeps[st.sweep];
+        sweep_tile<2>(c.phi, c.lam, sw, a.phases, a.ops, a.coef, a.hitab, a.grad_partials + st.partial, task,
+                      1u << (sw.k - kRegBits), smem, false);
+    } else if (st.kind == BSTEP_PAULI) {
+        batch_pauli(a, st, c, task, red);
+    } else {
+        batch_reduce(a, c, st.circuit, st.dep);
+    }
+}
+
+hipError_t launch_batch_level(hipStream_t st, const BatchArgs& a, uint32_t step_begin, uint32_t step_end,
+                              uint64_t task_begin, uint64_t n_tasks)
+{
+    if (!g_batch_ok) return hipErrorInvalidConfiguration;
+    const uint64_t max_grid = 1ull << 30;
+    for (uint64_t t0 = 0; t0 < n_tasks; t0 += max_grid) {
+        const uint64_t cnt = n_tasks - t0 < max_grid ? n_tasks - t0 : max_grid;
+        hipLaunchKernelGGL(k_batch, dim3(uint32_t(cnt)), dim3(kBatchThreads), batch_smem_bytes(), st, a, step_begin,
+                           step_end, task_begin + t0);
+    }
+    return hipGetLastError();
 }
 
 size_t sweep_smem_bytes(const SweepInfo& sw, int ns)
@@ -178,6 +310,11 @@ hipError_t init_sweep_kernels()
         }
     }
     g_max_smem = want;
+    // the batch kernel asks for exactly what it uses (it also has static LDS)
+    g_batch_ok = batch_smem_bytes() <= 64 * 1024 ||
+                 hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(batch_smem_bytes())) == hipSuccess;
+    (void)hipGetLastError();
     return hipSuccess;
 }
 

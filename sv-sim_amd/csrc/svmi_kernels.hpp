@@ -16,6 +16,53 @@ struct PauliTerm {
     double cre, cim;   // coeff * i^{|xmask & zmask|}
 };
 
+// ---- batched execution (one launch per dependency level) ----
+struct BatchStep {
+    uint32_t kind;        // BSTEP_*
+    uint32_t circuit;
+    int32_t dep;          // previous step of the same circuit (an earlier level), -1 = none
+    uint32_t sweep;       // FWD/ADJ: index into the sweep array
+    uint64_t task_begin;  // first queue task of this step
+    uint64_t n_tasks;
+    uint64_t partial;     // PAULI: first E partial; ADJ: first gradient partial
+    uint32_t init_zero;   // FWD: generate |0..0> instead of loading
+    uint32_t pad;
+};
+enum : uint32_t { BSTEP_FWD = 0, BSTEP_PAULI = 1, BSTEP_ADJ = 2, BSTEP_REDUCE = 3 };
+
+struct BatchCircuit {
+    c64* phi;             // state (forward result, then adjoint phi)
+    c64* lam;             // H psi, then adjoint lambda
+    uint32_t n;           // qubits of the device state
+    uint32_t group_begin, group_end;   // Pauli groups
+    uint32_t e_partial, n_e_partials;  // PAULI partials
+    uint32_t slot_begin, n_slots;      // gradient accumulators (global numbering)
+};
+
+struct BatchArgs {
+    const BatchStep* steps;
+    const BatchCircuit* circuits;
+    const SweepInfo* sweeps;
+    const PhaseInfo* phases;
+    const DOp* ops;
+    const c64* coef;
+    const uint64_t* hitab;
+    const PauliGroup* groups;
+    const PauliTerm* terms;
+    double* e_partials;
+    double* grad_partials;
+    double* gsum;         // per gradient accumulator
+    double* e_out;        // per circuit
+};
+
+constexpr uint32_t kPauliChunk = 4096;     // amplitudes per Pauli task
+
+size_t batch_smem_bytes();
+// one dependency level: the tasks [task_begin, task_begin + n_tasks) of the
+// steps [step_begin, step_end)
+hipError_t launch_batch_level(hipStream_t st, const BatchArgs& a, uint32_t step_begin, uint32_t step_end,
+                              uint64_t task_begin, uint64_t n_tasks);
+
 hipError_t init_sweep_kernels();
 size_t sweep_smem_limit();
 size_t sweep_smem_bytes(const SweepInfo& sw, int ns);

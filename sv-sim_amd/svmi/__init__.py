@@ -39,7 +39,7 @@ EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
            "svmi_set_state", "svmi_get_state", "svmi_get_amplitudes", "svmi_upload", "svmi_clear_circuit",
            "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version",
-           "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_plan"]
+           "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan"]
 
 
 class SvmiError(RuntimeError):
@@ -98,6 +98,7 @@ def lib():
                                          ctypes.c_int64, _vp, _vp]
         L.svmi_debug_emulate_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, _vp, _sz, ctypes.c_int,
                                                  ctypes.c_int, _vp, _vp, _vp, _vp]
+        L.svmi_debug_emulate_batch.argtypes = [_vp, _sz, _vp, _vp]
         L.svmi_debug_plan.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
         for name in EXPORTS:
@@ -260,9 +261,10 @@ def random_amplitudes(seed, idx):
     return r * np.cos(2 * np.pi * u1), r * np.sin(2 * np.pi * u1)
 
 
-def batch_adjoint(items, device=0):
+def batch_adjoint(items, device=0, _emulate=False):
     """items: list of dicts {n, gates, param_slots, n_params, terms}.
-    Returns (expvals, list of gradient arrays)."""
+    Returns (expvals, list of gradient arrays).  One launch per dependency level runs
+    every circuit's forward pass, <H> and adjoint sweep."""
     keep = []
     arr = (BatchItem * len(items))()
     total = 0
@@ -281,7 +283,10 @@ def batch_adjoint(items, device=0):
         total += int(it["n_params"])
     ev = np.zeros(len(items))
     gr = np.zeros(max(total, 1))
-    _check(lib().svmi_batch_adjoint(int(device), ctypes.cast(arr, _vp), len(items), _ptr(ev), _ptr(gr)))
+    if _emulate:
+        _check(lib().svmi_debug_emulate_batch(ctypes.cast(arr, _vp), len(items), _ptr(ev), _ptr(gr)))
+    else:
+        _check(lib().svmi_batch_adjoint(int(device), ctypes.cast(arr, _vp), len(items), _ptr(ev), _ptr(gr)))
     out, off = [], 0
     for it in items:
         out.append(gr[off:off + int(it["n_params"])].copy())
@@ -312,6 +317,11 @@ def emulate_adjoint(n, gates, terms, param_slots, n_params, state, tile_qubits=1
                                             int(tile_qubits), int(low_qubits), _ptr(re), _ptr(im),
                                             ctypes.byref(e), _ptr(grad) if n_params else None))
     return e.value, grad
+
+
+def emulate_batch(items):
+    """Host execution of batch_adjoint's task queue (planner check)."""
+    return batch_adjoint(items, _emulate=True)
 
 
 def plan(n, gates, fusion=True, tile_qubits=12, low_qubits=4, sweep_budget=0):

@@ -120,3 +120,24 @@ def test_invalid_gates_are_errors_not_aborts():
         with pytest.raises(svmi.SvmiError) as e:
             svmi.emulate(3, bad, st)
         assert e.value.status == status
+
+
+def test_batch_queue_emulation_matches_parameter_shift():
+    # the batched task queue (interleaved circuits, rebased programs,
+    # dependency chain, gradient reduction) executed on the host level by level,
+    # tasks of a level in reverse order
+    items = W.batch_ising(count=14, nmin=1, nmax=13, lmin=1, lmax=4, seed=21)
+    ev, grads = svmi.emulate_batch(items)
+    for it, e, g in zip(items, ev, grads):
+        n = it["n"]
+        assert abs(e - O.expval(n, *O.run(n, it["gates"]), it["terms"])) < 1e-12
+        gp = O.param_shift_grad(n, it["gates"], it["param_slots"], it["n_params"], it["terms"])
+        assert np.abs(g - gp).max() < 1e-12
+
+
+def test_batch_rejects_five_qubit_gates():
+    item = {"n": 6, "gates": [(svmi.OP["C4X"], [0, 1, 2, 3, 4])], "param_slots": [[-1, -1, -1]], "n_params": 0,
+            "terms": [(1.0, 0, 1)]}
+    with pytest.raises(svmi.SvmiError) as e:
+        svmi.emulate_batch([item])
+    assert e.value.status == 7
