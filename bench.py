@@ -157,6 +157,41 @@ def bench_batch(count=512):
             "speedup": t_loop / t_batch, "max_abs_diff": max(diff, float(np.abs(ev - ev2).max()))}
 
 
+def stream_copy_gbps(gib=4.0, reps=5):
+    """Measured device copy bandwidth (read + write bytes / time) of a
+    `gib`-GiB FP64 buffer, HIP events on the current stream, best of reps."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    best = 0.0
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2 * n * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del src, dst
+    torch.cuda.empty_cache()
+    return best
+
+
+def calibration_copy(gib):
+    """One device-to-device copy of `gib` GiB (a plain copy kernel of known
+    traffic: the counters of HBM reads and writes scale by it)."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] calibration copy of {n * 8} bytes done")
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def cpu_baseline_u3(n, budget_s=12.0, threads=None):
     """Oracle (OpenMP C restatement of the reference CPU engine) on the first
     gates of the same circuit, in chunks until ~budget_s of CPU work."""
@@ -186,11 +221,26 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
+    ap.add_argument("--only", choices=("qft", "adjoint", "batch"),
+                    help="run only this extra workload and print its result (profiling)")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="first run one device copy of this many GiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the engine runs on a single device)",
               file=sys.stderr)
         sys.exit(2)
+    # torch's HIP runtime initialises the device before the engine's
+    import torch  # noqa: F401
+    if a.copy_gib > 0:
+        calibration_copy(a.copy_gib)
+    if a.only:
+        fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch}[a.only]
+        log(f"[bench] only {a.only}")
+        print(json.dumps({a.only: fn()}), flush=True)
+        return
+    copy_gbps = stream_copy_gbps()
+    log(f"[bench] device copy {copy_gbps:.0f} GB/s")
     log(f"[bench] u3 brick-wall n={a.qubits} warmup={a.warmup} steps={a.steps}")
     r = bench_u3_brickwall(a.qubits, a.steps, a.warmup)
     ms = r["dt"] / a.steps * 1e3
@@ -200,7 +250,7 @@ def main():
     achieved = bytes_per_sweep / avg_sweep_s / 1e9
     log(f"[bench] {ms:.1f} ms/step, {r['passes']} passes/step, sweep avg {avg_sweep_s*1e3:.3f} ms, "
         f"{achieved:.0f} GB/s, sampling 4096 shots {r['sample_s']*1e3:.1f} ms")
-    extra = {}
+    extra = {"stream_copy_GBps": copy_gbps}
     if not a.quick:
         log("[bench] qft n=33 on a seeded random input")
         extra["qft_n33"] = bench_qft(33, steps=2)

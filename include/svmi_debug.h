@@ -37,6 +37,11 @@ svmi_status svmi_debug_emulate_adjoint(int n, const svmi_gate* gates, size_t n_g
 svmi_status svmi_debug_emulate_batch(const svmi_batch_item* items, size_t n_items,
                                      double* expvals, double* grads);
 
+/* Plan a batch exactly as svmi_batch_adjoint does, without executing it:
+ * number of dependency levels, device launches, tasks and state bytes. */
+svmi_status svmi_debug_plan_batch(const svmi_batch_item* items, size_t n_items, uint64_t* n_levels,
+                                  uint64_t* n_launches, uint64_t* n_tasks, uint64_t* state_bytes);
+
 /* Planner summary: number of full-state passes and a text description
  * (truncated to buflen-1 bytes; buf may be NULL). */
 svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates,

@@ -224,6 +224,10 @@ struct BatchPlan {
     uint64_t state_bytes = 0;
     std::vector<BatchStep> steps;        // level by level
     std::vector<uint32_t> level_begin;   // first step of each level, then steps.size()
+    std::vector<BatchLaunch> launches;   // in stream order
+    // final sums: partials = [gradient partials | <H> partials] ->
+    // results = [gradient accumulators (n_slots) | <H> per circuit]
+    std::vector<SumSegment> segs;
     uint64_t n_tasks = 0, n_e_partials = 0, n_grad_partials = 0;
     uint32_t n_slots = 0;
     std::vector<size_t> need_fill;       // circuits without forward sweeps
@@ -241,6 +245,28 @@ svmi_status verify_batch(const BatchPlan& bp)
         if (bp.level_begin[l] >= bp.level_begin[l + 1]) return fail(SVMI_ERR_STATE, "batch plan: empty level");
         for (uint32_t s = bp.level_begin[l]; s < bp.level_begin[l + 1]; s++) level_of[s] = uint32_t(l);
     }
+    // launches tile the step list in order; each holds steps of one level,
+    // one kind and one tile size, within the LDS the kernels may use
+    uint32_t next = 0;
+    for (const BatchLaunch& L : bp.launches) {
+        if (L.step_begin != next || L.step_end <= L.step_begin || L.step_end > bp.steps.size() ||
+            level_of[L.step_begin] != level_of[L.step_end - 1] ||
+            L.task_begin != bp.steps[L.step_begin].task_begin ||
+            batch_launch_smem(L.kind, L.k, L.max_grad) > size_t(160) * 1024)
+            return fail(SVMI_ERR_STATE, "batch plan: bad launch");
+        uint64_t nt = 0;
+        for (uint32_t s = L.step_begin; s < L.step_end; s++) {
+            const BatchStep& b = bp.steps[s];
+            const bool sweep = b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ;
+            if (b.kind != L.kind || (sweep && bp.sweeps[b.sweep].k != L.k) || b.init_zero != L.init ||
+                (b.kind == BSTEP_ADJ && bp.sweeps[b.sweep].n_grad > L.max_grad))
+                return fail(SVMI_ERR_STATE, "batch plan: mixed launch");
+            nt += b.n_tasks;
+        }
+        if (nt != L.n_tasks) return fail(SVMI_ERR_STATE, "batch plan: launch task count");
+        next = L.step_end;
+    }
+    if (next != bp.steps.size()) return fail(SVMI_ERR_STATE, "batch plan: steps outside launches");
     uint64_t t = 0;
     for (size_t s = 0; s < bp.steps.size(); s++) {
         const BatchStep& b = bp.steps[s];
@@ -275,6 +301,10 @@ svmi_status verify_batch(const BatchPlan& bp)
         }
     }
     if (t != bp.n_tasks) return fail(SVMI_ERR_STATE, "batch plan: task count");
+    for (const SumSegment& g : bp.segs)
+        if (g.len == 0 || g.src + g.len > bp.n_grad_partials + bp.n_e_partials ||
+            g.dst >= uint64_t(bp.n_slots) + bp.circs.size())
+            return fail(SVMI_ERR_STATE, "batch plan: bad sum segment");
     for (const BatchCircuit& c : bp.circs)
         if (c.group_end > bp.groups.size() || c.slot_begin + c.n_slots > bp.n_slots ||
             c.e_partial + c.n_e_partials > bp.n_e_partials)
@@ -411,12 +441,17 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
             bp.n_grad_partials += ad.sweeps[s].n_tiles * ad.sweeps[s].n_grad;
             cs.push_back(b);
         }
-        BatchStep rb;
-        std::memset(&rb, 0, sizeof(rb));
-        rb.kind = BSTEP_REDUCE;
-        rb.circuit = uint32_t(i);
-        rb.n_tasks = 1;
-        cs.push_back(rb);
+    }
+    for (size_t i = 0; i < n_items; i++) {
+        for (const BatchStep& b : csteps[i]) {
+            if (b.kind != BSTEP_ADJ) continue;
+            const SweepInfo& sw = bp.sweeps[b.sweep];
+            for (uint32_t g = 0; g < sw.n_grad; g++)
+                bp.segs.push_back({b.partial + uint64_t(g) * sw.n_tiles, sw.n_tiles,
+                                   uint64_t(bp.circs[i].slot_begin) + sw.grad_base + g});
+        }
+        const BatchCircuit& c = bp.circs[i];
+        bp.segs.push_back({bp.n_grad_partials + c.e_partial, c.n_e_partials, uint64_t(bp.n_slots) + i});
     }
     // level l = step l of every circuit that has one, largest circuits first
     std::vector<size_t> corder(n_items);
@@ -427,16 +462,51 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
     std::vector<std::vector<uint32_t>> idx_of(n_items);
     size_t max_levels = 0;
     for (auto& cs : csteps) max_levels = std::max(max_levels, cs.size());
+    // inside a level, steps of one kind and tile size are consecutive and
+    // form one launch
+    auto tile_k = [&](const BatchStep& b) -> uint32_t {
+        return (b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ) ? bp.sweeps[b.sweep].k : 0u;
+    };
     for (size_t lv = 0; lv < max_levels; lv++) {
         bp.level_begin.push_back(uint32_t(bp.steps.size()));
-        for (size_t i : corder) {
-            if (lv >= csteps[i].size()) continue;
+        std::vector<size_t> members;
+        for (size_t i : corder)
+            if (lv < csteps[i].size()) members.push_back(i);
+        std::stable_sort(members.begin(), members.end(), [&](size_t x, size_t y) {
+            const BatchStep &bx = csteps[x][lv], &by = csteps[y][lv];
+            if (bx.kind != by.kind) return bx.kind < by.kind;
+            return tile_k(bx) > tile_k(by);
+        });
+        for (size_t i : members) {
             BatchStep b = csteps[i][lv];
             b.dep = lv == 0 ? -1 : int32_t(idx_of[i][lv - 1]);
             b.task_begin = bp.n_tasks;
             bp.n_tasks += b.n_tasks;
-            idx_of[i].push_back(uint32_t(bp.steps.size()));
+            const uint32_t idx = uint32_t(bp.steps.size());
+            idx_of[i].push_back(idx);
             bp.steps.push_back(b);
+            const uint32_t k = tile_k(b);
+            const uint32_t ng = b.kind == BSTEP_ADJ ? bp.sweeps[b.sweep].n_grad : 0u;
+            if (!bp.launches.empty() && bp.launches.back().step_end == idx &&
+                bp.launches.back().step_begin >= bp.level_begin.back() && bp.launches.back().kind == b.kind &&
+                bp.launches.back().k == k && bp.launches.back().init == b.init_zero) {
+                BatchLaunch& L = bp.launches.back();
+                L.step_end = idx + 1;
+                L.n_tasks += b.n_tasks;
+                L.max_grad = std::max(L.max_grad, ng);
+            } else {
+                BatchLaunch L;
+                std::memset(&L, 0, sizeof(L));
+                L.step_begin = idx;
+                L.step_end = idx + 1;
+                L.kind = b.kind;
+                L.k = k;
+                L.max_grad = ng;
+                L.init = b.init_zero;
+                L.task_begin = b.task_begin;
+                L.n_tasks = b.n_tasks;
+                bp.launches.push_back(L);
+            }
         }
     }
     bp.level_begin.push_back(uint32_t(bp.steps.size()));
@@ -522,7 +592,7 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
             SVMI_HIP(launch_sweep(h->stream, prog.n_states, s0, s1, sw, dp.phases.as<PhaseInfo>(),
                                   dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials));
             if (partials)
-                SVMI_HIP(launch_reduce_grads(h->stream, partials, sw.n_tiles, sw.n_grad, grad_sums + sw.grad_base));
+                SVMI_HIP(launch_sum_rows(h->stream, partials, sw.n_grad, sw.n_tiles, grad_sums + sw.grad_base));
         } else {
             const StandaloneOp& so = prog.standalone[st.index];
             SVMI_HIP(launch_standalone(h->stream, s0, h->n_int, so, dp.coef.as<c64>()));
@@ -921,11 +991,10 @@ svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t 
     SVMI_HIP(hipSetDevice(device));
     SVMI_TRY(ensure_kernels());
 
-    DevBuf dstate, dsteps, dcircs, dsweeps, dphases, dops, dcoef, dhitab, dgroups, dterms, de_part, dg_part, dgsum,
-        de_out;
+    DevBuf dstate, dsteps, dcircs, dsweeps, dphases, dops, dcoef, dhitab, dgroups, dterms, dpart, dsegs, dout;
     auto release_all = [&]() {
         for (DevBuf* b : {&dstate, &dsteps, &dcircs, &dsweeps, &dphases, &dops, &dcoef, &dhitab, &dgroups, &dterms,
-                          &de_part, &dg_part, &dgsum, &de_out})
+                          &dpart, &dsegs, &dout})
             b->release();
     };
     hipStream_t st = nullptr;
@@ -962,10 +1031,9 @@ svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t 
         if ((rc = put(dhitab, bp.hitab.data(), bp.hitab.size() * sizeof(uint64_t))) != SVMI_OK) break;
         if ((rc = put(dgroups, bp.groups.data(), bp.groups.size() * sizeof(PauliGroup))) != SVMI_OK) break;
         if ((rc = put(dterms, bp.terms.data(), bp.terms.size() * sizeof(PauliTerm))) != SVMI_OK) break;
-        if ((rc = de_part.ensure(std::max<uint64_t>(bp.n_e_partials, 1) * sizeof(double))) != SVMI_OK) break;
-        if ((rc = dg_part.ensure(std::max<uint64_t>(bp.n_grad_partials, 1) * sizeof(double))) != SVMI_OK) break;
-        if ((rc = dgsum.ensure(std::max<uint32_t>(bp.n_slots, 1) * sizeof(double))) != SVMI_OK) break;
-        if ((rc = de_out.ensure(n_items * sizeof(double))) != SVMI_OK) break;
+        if ((rc = dpart.ensure((bp.n_grad_partials + bp.n_e_partials) * sizeof(double))) != SVMI_OK) break;
+        if ((rc = put(dsegs, bp.segs.data(), bp.segs.size() * sizeof(SumSegment))) != SVMI_OK) break;
+        if ((rc = dout.ensure((bp.n_slots + n_items) * sizeof(double))) != SVMI_OK) break;
         BatchArgs a;
         a.steps = dsteps.as<BatchStep>();
         a.circuits = dcircs.as<BatchCircuit>();
@@ -976,30 +1044,24 @@ svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t 
         a.hitab = dhitab.as<uint64_t>();
         a.groups = dgroups.as<PauliGroup>();
         a.terms = dterms.as<PauliTerm>();
-        a.e_partials = de_part.as<double>();
-        a.grad_partials = dg_part.as<double>();
-        a.gsum = dgsum.as<double>();
-        a.e_out = de_out.as<double>();
+        a.grad_partials = dpart.as<double>();
+        a.e_partials = dpart.as<double>() + bp.n_grad_partials;
         hipError_t e = hipSuccess;
-        for (size_t l = 0; l + 1 < bp.level_begin.size() && e == hipSuccess; l++) {
-            const uint32_t s0 = bp.level_begin[l], s1 = bp.level_begin[l + 1];
-            const uint64_t t0 = bp.steps[s0].task_begin;
-            const uint64_t t1 = bp.steps[s1 - 1].task_begin + bp.steps[s1 - 1].n_tasks;
-            e = launch_batch_level(st, a, s0, s1, t0, t1 - t0);
-        }
+        for (size_t l = 0; l < bp.launches.size() && e == hipSuccess; l++) e = launch_batch(st, a, bp.launches[l]);
+        if (e == hipSuccess)
+            e = launch_sum_segments(st, dpart.as<double>(), dsegs.as<SumSegment>(), bp.segs.size(), dout.as<double>());
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
             rc = fail(SVMI_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
             break;
         }
-        std::vector<double> gs(bp.n_slots), eo(n_items);
-        if (bp.n_slots)
-            e = hipMemcpy(gs.data(), dgsum.p, bp.n_slots * sizeof(double), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(eo.data(), de_out.p, n_items * sizeof(double), hipMemcpyDeviceToHost);
+        std::vector<double> res(bp.n_slots + n_items);
+        e = hipMemcpy(res.data(), dout.p, res.size() * sizeof(double), hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
             rc = fail(SVMI_ERR_DEVICE, std::string("batch results: ") + hipGetErrorString(e));
             break;
         }
+        const std::vector<double> gs(res.begin(), res.begin() + bp.n_slots), eo(res.begin() + bp.n_slots, res.end());
         finish_batch(bp, items, n_items, gs, eo, expvals, grads);
     } while (false);
     if (st) {
@@ -1274,11 +1336,13 @@ svmi_status svmi_debug_emulate_batch(const svmi_batch_item* items, size_t n_item
         c64* p = phi(i);
         for (uint64_t x = 0; x < (1ull << bp.circs[i].n); x++) p[x] = c64{x == 0 ? 1.0 : 0.0, 0.0};
     }
-    std::vector<double> e_part(bp.n_e_partials), g_part(bp.n_grad_partials), gs(bp.n_slots), eo(n_items);
-    // level by level; inside a level the tasks run last to first, since the
-    // device runs them in no particular order
-    for (size_t l = 0; l + 1 < bp.level_begin.size(); l++)
-    for (uint32_t si = bp.level_begin[l + 1]; si-- > bp.level_begin[l];) {
+    std::vector<double> part(bp.n_grad_partials + bp.n_e_partials);
+    double* g_part = part.data();
+    double* e_part = part.data() + bp.n_grad_partials;
+    // launch by launch; inside a launch the tasks run last to first, since
+    // the device runs them in no particular order
+    for (const BatchLaunch& L : bp.launches)
+    for (uint32_t si = L.step_end; si-- > L.step_begin;) {
         const BatchStep& st = bp.steps[si];
         const BatchCircuit& c = bp.circs[st.circuit];
         c64* ph = phi(st.circuit);
@@ -1289,9 +1353,9 @@ svmi_status svmi_debug_emulate_batch(const svmi_batch_item* items, size_t n_item
                 std::vector<double> g(sw.n_grad);
                 host_tile(sw, bp.phases.data(), bp.ops.data(), bp.coef.data(), bp.hitab.data(),
                           st.kind == BSTEP_ADJ ? 2 : 1, ph, st.kind == BSTEP_ADJ ? lm : nullptr, task,
-                          st.init_zero != 0, g.data());
-                for (uint32_t i = 0; i < sw.n_grad; i++) g_part[st.partial + task * sw.n_grad + i] = g[i];
-            } else if (st.kind == BSTEP_PAULI) {
+                          L.init != 0, g.data());
+                for (uint32_t i = 0; i < sw.n_grad; i++) g_part[st.partial + i * sw.n_tiles + task] = g[i];
+            } else {
                 const uint64_t dim = 1ull << c.n;
                 const uint64_t y0 = task * kPauliChunk, y1 = std::min<uint64_t>(y0 + kPauliChunk, dim);
                 double acc = 0.0;
@@ -1312,23 +1376,42 @@ svmi_status svmi_debug_emulate_batch(const svmi_batch_item* items, size_t n_item
                     acc += ph[y].re * l.re + ph[y].im * l.im;
                 }
                 e_part[st.partial + task] = acc;
-            } else {
-                for (int32_t s2 = st.dep; s2 >= 0 && bp.steps[s2].kind == BSTEP_ADJ; s2 = bp.steps[s2].dep) {
-                    const BatchStep& as = bp.steps[s2];
-                    const SweepInfo& sw = bp.sweeps[as.sweep];
-                    for (uint32_t i = 0; i < sw.n_grad; i++) {
-                        double v = 0.0;
-                        for (uint64_t t = 0; t < sw.n_tiles; t++) v += g_part[as.partial + t * sw.n_grad + i];
-                        gs[c.slot_begin + sw.grad_base + i] = v;
-                    }
-                }
-                double e = 0.0;
-                for (uint32_t j = 0; j < c.n_e_partials; j++) e += e_part[c.e_partial + j];
-                eo[st.circuit] = e;
             }
         }
     }
+    // final segment sums, in the device kernel's order (256 strided
+    // partial sums, then a halving tree)
+    std::vector<double> res(bp.n_slots + n_items, 0.0);
+    for (const SumSegment& g : bp.segs) {
+        double red[256];
+        for (uint32_t t = 0; t < 256; t++) {
+            double acc = 0.0;
+            for (uint64_t i = t; i < g.len; i += 256) acc += part[g.src + i];
+            red[t] = acc;
+        }
+        for (uint32_t k = 128; k > 0; k >>= 1)
+            for (uint32_t t = 0; t < k; t++) red[t] += red[t + k];
+        res[g.dst] = red[0];
+    }
+    const std::vector<double> gs(res.begin(), res.begin() + bp.n_slots), eo(res.begin() + bp.n_slots, res.end());
     finish_batch(bp, items, n_items, gs, eo, expvals, grads);
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_plan_batch(const svmi_batch_item* items, size_t n_items, uint64_t* n_levels,
+                                  uint64_t* n_launches, uint64_t* n_tasks, uint64_t* state_bytes)
+{
+    if (n_items == 0) return fail(SVMI_ERR_ARG, "empty batch");
+    std::vector<double> ev(n_items), gr(1);
+    size_t total = 0;
+    for (size_t i = 0; i < n_items; i++) total += items ? items[i].n_params : 0;
+    gr.resize(std::max<size_t>(total, 1));
+    BatchPlan bp;
+    SVMI_TRY(build_batch(items, n_items, ev.data(), gr.data(), bp));
+    if (n_levels) *n_levels = bp.level_begin.size() - 1;
+    if (n_launches) *n_launches = bp.launches.size();
+    if (n_tasks) *n_tasks = bp.n_tasks;
+    if (state_bytes) *state_bytes = bp.state_bytes;
     return SVMI_OK;
 }
 

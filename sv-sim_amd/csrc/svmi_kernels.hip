@@ -20,9 +20,11 @@ namespace svmi {
 // ------------------------------------------------------------------ sweeps
 
 // One tile of one sweep.  Threads tid < nt (= 2^(k-4)) own one register
-// group per phase; the others (batched kernel: blockDim > nt) only join the
-// barriers.  init_zero: the tile is generated as |0..0> instead of loaded.
-template <int NS>
+// group per phase; with kIdle (batched kernel: blockDim >= nt) the others
+// only join the barriers.  Without kIdle every thread is active, which keeps
+// the phase body free of divergence (and its register groups out of
+// scratch).  init_zero: the tile is generated as |0..0> instead of loaded.
+template <int NS, bool kIdle>
 __device__ __attribute__((always_inline)) inline void
 sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
            const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
@@ -37,7 +39,7 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     double* gacc = reinterpret_cast<double*>(t0 + tsize * NS);
 
     const uint32_t tid = threadIdx.x;
-    const bool active = tid < nt;
+    const bool active = !kIdle || tid < nt;
     const uint32_t nwaves = (nt + 63) / 64;
     const uint64_t base = tile_base(sw, tile);
     const uint32_t cmask = (1u << sw.c) - 1u;
@@ -106,11 +108,13 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
             if (NS == 2) s1[g] = t1[lds_slot(l)];
         }
     }
+    // gradient partials are stored grad-major ([i][tile]) so the sum over
+    // tiles of one gradient reads one contiguous segment
     if (NS == 2) {
         for (uint32_t i = tid; i < sw.n_grad; i += blockDim.x) {
             double v = 0.0;
             for (uint32_t w = 0; w < nwaves; w++) v += gacc[i * nwaves + w];
-            grad_partials[tile * sw.n_grad + i] = v;
+            grad_partials[i * sw.n_tiles + tile] = v;
         }
     }
 }
@@ -122,8 +126,8 @@ k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__
          uint64_t tile_offset)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    sweep_tile<1>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x, smem,
-                  false);
+    sweep_tile<1, false>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x,
+                         smem, false);
 }
 
 // Two states (adjoint): one wave per SIMD, registers for both groups.
@@ -133,32 +137,77 @@ k_sweep2(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw, const P
          double* __restrict__ grad_partials, uint64_t tile_offset)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    sweep_tile<2>(s0, s1, sw, phases, ops, coef, hitab, grad_partials, tile_offset + blockIdx.x, blockDim.x, smem,
-                  false);
+    sweep_tile<2, false>(s0, s1, sw, phases, ops, coef, hitab, grad_partials, tile_offset + blockIdx.x, blockDim.x,
+                         smem, false);
 }
 
 static size_t g_max_smem = 64 * 1024;  // dynamic LDS the sweep kernels may use
-static bool g_batch_ok = false;         // batch kernel's LDS attribute was accepted
 
 // --------------------------------------------------------------- batching
 //
-// A batch is a queue of steps (forward sweep, Pauli step, adjoint sweep,
+// A batch is a list of steps (forward sweep, Pauli step, adjoint sweep,
 // reduction) of many circuits; a task is one tile (or chunk) of one step.
-// Steps are grouped into dependency levels: level l holds step l of every
-// circuit that has one, so the tasks of one level are independent of each
-// other.  One launch covers one level, blockIdx.x -> task; stream order
-// between the level launches supplies every dependency, so no workgroup
-// ever waits for another one.
+// Level l holds step l of every circuit that has one, so the tasks of a
+// level are independent; inside a level the steps are grouped by kind and
+// tile size, and each group is one launch (blockIdx.x -> task, blockDim =
+// the group's tile threads).  Stream order between the launches supplies
+// every dependency, so no workgroup ever waits for another one.
 
-constexpr uint32_t kBatchThreads = 256;
-constexpr size_t kBatchSmem = size_t(16) << kMaxTileQubits;   // 64 KiB: one 2^12 tile or two 2^11 tiles
-constexpr size_t kBatchGradLds = size_t(8) * kMaxGradsPerSweep * 4;
-
-size_t batch_smem_bytes() { return kBatchSmem + kBatchGradLds; }
-
-__device__ void batch_pauli(const BatchArgs& a, const BatchStep& st, const BatchCircuit& c, uint64_t task,
-                            double* red)
+// The step of task t among steps [s0, s1): the last one whose task_begin is
+// not above t.
+__device__ inline uint32_t batch_step_of(const BatchStep* __restrict__ steps, uint32_t s0, uint32_t s1, uint64_t t)
 {
+    uint32_t lo = s0, hi = s1;
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (steps[mid].task_begin <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Read-only program tables of a batch, as restrict kernel arguments so the
+// tile code reads op headers through the scalar path as k_sweep does.
+#define SVMI_BATCH_TABLES                                                                            \
+    const BatchStep* __restrict__ steps, const BatchCircuit* __restrict__ circuits,                  \
+        const SweepInfo* __restrict__ sweeps, const PhaseInfo* __restrict__ phases,                 \
+        const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab
+
+// kInit: the steps are first forward sweeps (tiles generated as |0..0>).
+template <bool kInit>
+__global__ void __launch_bounds__(kThreads, 2)
+k_batch_fwd(SVMI_BATCH_TABLES, uint32_t s0, uint32_t s1, uint64_t task0)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t t = task0 + blockIdx.x;
+    const BatchStep st = steps[batch_step_of(steps, s0, s1, t)];
+    const SweepInfo sw = sweeps[st.sweep];
+    sweep_tile<1, false>(circuits[st.circuit].phi, nullptr, sw, phases, ops, coef, hitab, nullptr,
+                         t - st.task_begin, blockDim.x, smem, kInit);
+}
+
+__global__ void __launch_bounds__(kThreads, 1)
+k_batch_adj(SVMI_BATCH_TABLES, double* __restrict__ grad_partials, uint32_t s0, uint32_t s1, uint64_t task0)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint64_t t = task0 + blockIdx.x;
+    const BatchStep st = steps[batch_step_of(steps, s0, s1, t)];
+    const SweepInfo sw = sweeps[st.sweep];
+    const BatchCircuit& c = circuits[st.circuit];
+    sweep_tile<2, false>(c.phi, c.lam, sw, phases, ops, coef, hitab, grad_partials + st.partial, t - st.task_begin,
+                         blockDim.x, smem, false);
+}
+
+// <H> step: lam = H phi on a chunk of kPauliChunk amplitudes and the chunk's
+// partial sum of Re <phi|lam>.
+__global__ void __launch_bounds__(kThreads)
+k_batch_pauli(BatchArgs a, uint32_t s0, uint32_t s1, uint64_t task0)
+{
+    __shared__ double red[kThreads];
+    const uint64_t t = task0 + blockIdx.x;
+    const BatchStep st = a.steps[batch_step_of(a.steps, s0, s1, t)];
+    const BatchCircuit& c = a.circuits[st.circuit];
+    const uint64_t task = t - st.task_begin;
     const uint64_t dim = 1ull << c.n;
     const uint64_t y0 = task * kPauliChunk;
     const uint64_t y1 = y0 + kPauliChunk < dim ? y0 + kPauliChunk : dim;
@@ -169,8 +218,8 @@ __device__ void batch_pauli(const BatchArgs& a, const BatchStep& st, const Batch
             const PauliGroup G = a.groups[g];
             const uint64_t x = y ^ G.xmask;
             double fr = 0.0, fi = 0.0;
-            for (uint32_t t = G.begin; t < G.end; t++) {
-                const PauliTerm T = a.terms[t];
+            for (uint32_t e = G.begin; e < G.end; e++) {
+                const PauliTerm T = a.terms[e];
                 const bool neg = __popcll(x & T.zmask) & 1;
                 fr += neg ? -T.cre : T.cre;
                 fi += neg ? -T.cim : T.cim;
@@ -190,69 +239,44 @@ __device__ void batch_pauli(const BatchArgs& a, const BatchStep& st, const Batch
     if (threadIdx.x == 0) a.e_partials[st.partial + task] = red[0];
 }
 
-__device__ void batch_reduce(const BatchArgs& a, const BatchCircuit& c, uint32_t ci, int32_t last)
+size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad)
 {
-    // gradient accumulators of the circuit's adjoint sweeps (found by walking
-    // its dependency chain back from the last one), each summed over its
-    // tiles in tile order; then <H> from the Pauli partials
-    for (int32_t s = last; s >= 0 && a.steps[s].kind == BSTEP_ADJ; s = a.steps[s].dep) {
-        const BatchStep st = a.steps[s];
-        const SweepInfo& sw = a.sweeps[st.sweep];
-        for (uint32_t i = threadIdx.x; i < sw.n_grad; i += blockDim.x) {
-            double v = 0.0;
-            for (uint64_t t = 0; t < sw.n_tiles; t++) v += a.grad_partials[st.partial + t * sw.n_grad + i];
-            a.gsum[c.slot_begin + sw.grad_base + i] = v;
-        }
+    if (kind == BSTEP_FWD) return size_t(16) << k;
+    if (kind == BSTEP_ADJ) {
+        const size_t nwaves = ((size_t(1) << (k - kRegBits)) + 63) / 64;
+        return (size_t(32) << k) + size_t(8) * max_grad * nwaves;
     }
-    if (threadIdx.x == 0) {
-        double e = 0.0;
-        for (uint32_t j = 0; j < c.n_e_partials; j++) e += a.e_partials[c.e_partial + j];
-        a.e_out[ci] = e;
-    }
+    return 0;
 }
 
-// One task of the level made of steps [step_begin, step_end): the task
-// number is task0 + blockIdx.x, its step the last one whose task_begin is
-// not above it.
-__global__ void __launch_bounds__(kBatchThreads, 1)
-k_batch(BatchArgs a, uint32_t step_begin, uint32_t step_end, uint64_t task0)
+hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double red[kBatchThreads];
-    const uint64_t t = task0 + blockIdx.x;
-    uint32_t lo = step_begin, hi = step_end;
-    while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (a.steps[mid].task_begin <= t) lo = mid;
-        else hi = mid;
-    }
-    const BatchStep st = a.steps[lo];
-    const uint64_t task = t - st.task_begin;
-    const BatchCircuit& c = a.circuits[st.circuit];
-    if (st.kind == BSTEP_FWD) {
-        const SweepInfo& sw = a.sweeps[st.sweep];
-        sweep_tile<1>(c.phi, nullptr, sw, a.phases, a.ops, a.coef, a.hitab, nullptr, task,
-                      1u << (sw.k - kRegBits), smem, st.init_zero != 0);
-    } else if (st.kind == BSTEP_ADJ) {
-        const SweepInfo& sw = a.sweeps[st.sweep];
-        sweep_tile<2>(c.phi, c.lam, sw, a.phases, a.ops, a.coef, a.hitab, a.grad_partials + st.partial, task,
-                      1u << (sw.k - kRegBits), smem, false);
-    } else if (st.kind == BSTEP_PAULI) {
-        batch_pauli(a, st, c, task, red);
-    } else {
-        batch_reduce(a, c, st.circuit, st.dep);
-    }
-}
-
-hipError_t launch_batch_level(hipStream_t st, const BatchArgs& a, uint32_t step_begin, uint32_t step_end,
-                              uint64_t task_begin, uint64_t n_tasks)
-{
-    if (!g_batch_ok) return hipErrorInvalidConfiguration;
+    const size_t smem = batch_launch_smem(L.kind, L.k, L.max_grad);
+    if (smem > g_max_smem) return hipErrorInvalidConfiguration;
+    const uint32_t threads = (L.kind == BSTEP_FWD || L.kind == BSTEP_ADJ) ? 1u << (L.k - kRegBits) : kThreads;
     const uint64_t max_grid = 1ull << 30;
-    for (uint64_t t0 = 0; t0 < n_tasks; t0 += max_grid) {
-        const uint64_t cnt = n_tasks - t0 < max_grid ? n_tasks - t0 : max_grid;
-        hipLaunchKernelGGL(k_batch, dim3(uint32_t(cnt)), dim3(kBatchThreads), batch_smem_bytes(), st, a, step_begin,
-                           step_end, task_begin + t0);
+    for (uint64_t t0 = 0; t0 < L.n_tasks; t0 += max_grid) {
+        const uint32_t cnt = uint32_t(L.n_tasks - t0 < max_grid ? L.n_tasks - t0 : max_grid);
+        const uint64_t first = L.task_begin + t0;
+        switch (L.kind) {
+        case BSTEP_FWD:
+            if (L.init)
+                hipLaunchKernelGGL(k_batch_fwd<true>, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits,
+                                   a.sweeps, a.phases, a.ops, a.coef, a.hitab, L.step_begin, L.step_end, first);
+            else
+                hipLaunchKernelGGL(k_batch_fwd<false>, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits,
+                                   a.sweeps, a.phases, a.ops, a.coef, a.hitab, L.step_begin, L.step_end, first);
+            break;
+        case BSTEP_ADJ:
+            hipLaunchKernelGGL(k_batch_adj, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits, a.sweeps,
+                               a.phases, a.ops, a.coef, a.hitab, a.grad_partials, L.step_begin, L.step_end, first);
+            break;
+        case BSTEP_PAULI:
+            hipLaunchKernelGGL(k_batch_pauli, dim3(cnt), dim3(threads), 0, st, a, L.step_begin, L.step_end, first);
+            break;
+        default:
+            return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }
@@ -309,12 +333,17 @@ hipError_t init_sweep_kernels()
             want = 64 * 1024;
         }
     }
+    if (want > 64 * 1024 &&
+        (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<true>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<false>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_adj), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             int(want)) != hipSuccess)) {
+        (void)hipGetLastError();
+        want = 64 * 1024;
+    }
     g_max_smem = want;
-    // the batch kernel asks for exactly what it uses (it also has static LDS)
-    g_batch_ok = batch_smem_bytes() <= 64 * 1024 ||
-                 hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(batch_smem_bytes())) == hipSuccess;
-    (void)hipGetLastError();
     return hipSuccess;
 }
 
@@ -680,21 +709,60 @@ hipError_t launch_pauli(hipStream_t st, const c64* psi, c64* lam, uint64_t dim, 
 }
 
 // Sum of per-tile gradient partials, one thread per accumulator (fixed order).
-__global__ void k_reduce_grads(const double* __restrict__ partials, uint64_t ntiles, uint32_t ngrad,
-                               double* __restrict__ out)
+// ------------------------------------------------------- segment sums
+//
+// dst[seg.dst] = sum of src[seg.src .. seg.src + seg.len): one workgroup per
+// segment; thread t adds elements t, t + 256, ... in order, then a fixed
+// tree over the threads -- the same order on every run.
+
+__global__ void __launch_bounds__(kThreads)
+k_sum_segments(const double* __restrict__ src, const SumSegment* __restrict__ segs, double* __restrict__ dst)
 {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngrad) return;
-    double s = 0.0;
-    for (uint64_t t = 0; t < ntiles; t++) s += partials[t * ngrad + g];
-    out[g] = s;
+    __shared__ double red[kThreads];
+    const SumSegment g = segs[blockIdx.x];
+    double acc = 0.0;
+    for (uint64_t i = threadIdx.x; i < g.len; i += blockDim.x) acc += src[g.src + i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[g.dst] = red[0];
 }
 
-hipError_t launch_reduce_grads(hipStream_t st, const double* partials, uint64_t ntiles, uint32_t ngrad,
-                               double* out)
+// Rows of a [rows][len] array: dst[r] = sum of row r (same order as above).
+__global__ void __launch_bounds__(kThreads)
+k_sum_rows(const double* __restrict__ src, uint64_t len, double* __restrict__ dst)
 {
-    if (ngrad == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_grads, dim3((ngrad + 63) / 64), dim3(64), 0, st, partials, ntiles, ngrad, out);
+    __shared__ double red[kThreads];
+    const double* row = src + uint64_t(blockIdx.x) * len;
+    double acc = 0.0;
+    for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) acc += row[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[blockIdx.x] = red[0];
+}
+
+hipError_t launch_sum_segments(hipStream_t st, const double* src, const SumSegment* segs, uint64_t nseg,
+                               double* dst)
+{
+    const uint64_t max_grid = 1ull << 30;
+    for (uint64_t s0 = 0; s0 < nseg; s0 += max_grid) {
+        const uint32_t cnt = uint32_t(nseg - s0 < max_grid ? nseg - s0 : max_grid);
+        hipLaunchKernelGGL(k_sum_segments, dim3(cnt), dim3(kThreads), 0, st, src, segs + s0, dst);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_rows(hipStream_t st, const double* src, uint32_t rows, uint64_t len, double* dst)
+{
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_rows, dim3(rows), dim3(kThreads), 0, st, src, len, dst);
     return hipGetLastError();
 }
 

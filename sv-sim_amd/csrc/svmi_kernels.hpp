@@ -28,7 +28,7 @@ struct BatchStep {
     uint32_t init_zero;   // FWD: generate |0..0> instead of loading
     uint32_t pad;
 };
-enum : uint32_t { BSTEP_FWD = 0, BSTEP_PAULI = 1, BSTEP_ADJ = 2, BSTEP_REDUCE = 3 };
+enum : uint32_t { BSTEP_FWD = 0, BSTEP_PAULI = 1, BSTEP_ADJ = 2 };
 
 struct BatchCircuit {
     c64* phi;             // state (forward result, then adjoint phi)
@@ -51,17 +51,23 @@ struct BatchArgs {
     const PauliTerm* terms;
     double* e_partials;
     double* grad_partials;
-    double* gsum;         // per gradient accumulator
-    double* e_out;        // per circuit
 };
 
 constexpr uint32_t kPauliChunk = 4096;     // amplitudes per Pauli task
 
-size_t batch_smem_bytes();
-// one dependency level: the tasks [task_begin, task_begin + n_tasks) of the
-// steps [step_begin, step_end)
-hipError_t launch_batch_level(hipStream_t st, const BatchArgs& a, uint32_t step_begin, uint32_t step_end,
-                              uint64_t task_begin, uint64_t n_tasks);
+
+// One launch of a batch: the steps [step_begin, step_end) of one level,
+// all of one kind and (sweeps) one tile size k; their tasks are
+// [task_begin, task_begin + n_tasks).
+struct BatchLaunch {
+    uint32_t step_begin, step_end;
+    uint32_t kind, k;
+    uint32_t max_grad;      // ADJ: largest n_grad of the group's sweeps
+    uint32_t init;          // FWD: first sweeps (tiles start as |0..0>)
+    uint64_t task_begin, n_tasks;
+};
+size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad);
+hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L);
 
 hipError_t init_sweep_kernels();
 size_t sweep_smem_limit();
@@ -84,7 +90,14 @@ hipError_t launch_sample(hipStream_t st, const c64* s, uint32_t nthr, uint32_t c
 uint32_t pauli_grid(uint64_t dim);
 hipError_t launch_pauli(hipStream_t st, const c64* psi, c64* lam, uint64_t dim, const PauliGroup* groups,
                         uint32_t ngroups, const PauliTerm* terms, double* partials);
-hipError_t launch_reduce_grads(hipStream_t st, const double* partials, uint64_t ntiles, uint32_t ngrad,
-                               double* out);
+// dst[seg.dst] = sum of src[seg.src, seg.src + seg.len), per segment, in a
+// fixed order (one workgroup per segment).
+struct SumSegment {
+    uint64_t src, len, dst;
+};
+hipError_t launch_sum_segments(hipStream_t st, const double* src, const SumSegment* segs, uint64_t nseg,
+                               double* dst);
+// dst[r] = sum of src[r * len, (r + 1) * len), r < rows, same order.
+hipError_t launch_sum_rows(hipStream_t st, const double* src, uint32_t rows, uint64_t len, double* dst);
 
 }  // namespace svmi

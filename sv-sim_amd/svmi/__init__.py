@@ -8,6 +8,7 @@ expectations and adjoint gradients.  Gate records use the reference layout
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -39,7 +40,8 @@ EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
            "svmi_set_state", "svmi_get_state", "svmi_get_amplitudes", "svmi_upload", "svmi_clear_circuit",
            "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version",
-           "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan"]
+           "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
+           "svmi_debug_plan_batch"]
 
 
 class SvmiError(RuntimeError):
@@ -67,12 +69,23 @@ _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
 
 
+def _torch_runtime_first():
+    """PyTorch-ROCm wheels carry their own HIP runtime; libsvmi.so uses the
+    system one.  Both work in one process when torch's runtime initialises
+    the device first (the other order leaves torch without GPUs), so if torch
+    is loaded, initialise its device runtime before libsvmi.so."""
+    t = sys.modules.get("torch")
+    if t is not None and t.cuda.is_available():
+        t.cuda.init()
+
+
 def lib():
     """Load libsvmi.so (raises if it was not built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} not built; run python sv-sim_amd/build.py")
+        _torch_runtime_first()
         L = ctypes.CDLL(LIB_PATH)
         st = ctypes.c_int
         L.svmi_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
@@ -261,10 +274,7 @@ def random_amplitudes(seed, idx):
     return r * np.cos(2 * np.pi * u1), r * np.sin(2 * np.pi * u1)
 
 
-def batch_adjoint(items, device=0, _emulate=False):
-    """items: list of dicts {n, gates, param_slots, n_params, terms}.
-    Returns (expvals, list of gradient arrays).  One launch per dependency level runs
-    every circuit's forward pass, <H> and adjoint sweep."""
+def _batch_items(items):
     keep = []
     arr = (BatchItem * len(items))()
     total = 0
@@ -281,6 +291,14 @@ def batch_adjoint(items, device=0, _emulate=False):
         arr[i].terms = t.ctypes.data if t.size else None
         arr[i].n_terms = len(t)
         total += int(it["n_params"])
+    return arr, keep, total
+
+
+def batch_adjoint(items, device=0, _emulate=False):
+    """items: list of dicts {n, gates, param_slots, n_params, terms}.
+    Returns (expvals, list of gradient arrays).  One launch per dependency
+    level runs every circuit's forward pass, <H> and adjoint sweep."""
+    arr, keep, total = _batch_items(items)
     ev = np.zeros(len(items))
     gr = np.zeros(max(total, 1))
     if _emulate:
@@ -295,6 +313,13 @@ def batch_adjoint(items, device=0, _emulate=False):
 
 
 # ---- host checks of the planner (no device; see include/svmi_debug.h) ----
+
+def plan_batch(items):
+    """{levels, launches, tasks, state_bytes} of the batch plan."""
+    arr, keep, _ = _batch_items(items)
+    v = [ctypes.c_uint64() for _ in range(4)]
+    _check(lib().svmi_debug_plan_batch(ctypes.cast(arr, _vp), len(items), *[ctypes.byref(x) for x in v]))
+    return dict(zip(("levels", "launches", "tasks", "state_bytes"), (x.value for x in v)))
 
 def emulate(n, gates, state, fusion=True, tile_qubits=12, low_qubits=4, sweep_budget=0):
     g = as_gates(gates)
