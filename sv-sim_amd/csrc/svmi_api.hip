@@ -546,6 +546,14 @@ struct svmi_sim {
     bool profile = false;
     std::vector<hipEvent_t> prof_events;
     svmi_stats stats{};
+    // backward program of the uploaded circuit for the last parameter map
+    // (reused while the circuit, the map and the options stay the same)
+    bool adj_valid = false;
+    std::vector<int32_t> adj_slots;
+    size_t adj_n_params = 0;
+    Program adj_prog;
+    DeviceProgram adj_dprog;
+    DevBuf adj_gsum;
 };
 
 namespace {
@@ -702,6 +710,8 @@ svmi_status svmi_destroy(svmi_handle h)
     for (auto& b : h->scratch) b.release();
     for (hipEvent_t e : h->prof_events) (void)hipEventDestroy(e);
     h->dprog.release();
+    h->adj_dprog.release();
+    h->adj_gsum.release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -815,6 +825,7 @@ svmi_status svmi_upload(svmi_handle h, const svmi_gate* gates, size_t n_gates)
     SVMI_TRY(use_device(h));
     SVMI_TRY(check_gates(gates, n_gates, h->n));
     h->gates.assign(gates, gates + n_gates);
+    h->adj_valid = false;
     plan_forward(gates, n_gates, h->n_int, h->opt, h->prog);
     SVMI_TRY(h->dprog.upload(h->prog, h->stream));
     h->uploaded = true;
@@ -829,6 +840,7 @@ svmi_status svmi_clear_circuit(svmi_handle h)
     h->gates.clear();
     h->prog = Program();
     h->uploaded = false;
+    h->adj_valid = false;
     return SVMI_OK;
 }
 
@@ -950,29 +962,30 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
     SVMI_TRY(pauli_device(h, h->sv, lam, terms, n_terms, &E));
     *expval_out = E;
     for (size_t p = 0; p < n_params; p++) grad_out[p] = 0.0;
-    // backward
-    std::vector<AdjEntry> entries;
-    adjoint_entries(h->gates.data(), h->gates.size(), param_slots, entries);
-    Program bp;
-    build_adjoint_program(entries, h->n_int, h->opt, bp);
-    DeviceProgram dbp;
-    svmi_status st = dbp.upload(bp, h->stream);
-    DevBuf gsum;
-    if (st == SVMI_OK) st = gsum.ensure(std::max<size_t>(bp.n_grad, 1) * sizeof(double));
-    if (st == SVMI_OK) st = exec_program(h, bp, dbp, phi, lam, gsum.as<double>());
+    // backward (planned once per circuit and parameter map)
+    const size_t n_slots = 3 * h->gates.size();
+    if (!h->adj_valid || h->adj_n_params != n_params ||
+        !std::equal(param_slots, param_slots + n_slots, h->adj_slots.begin(), h->adj_slots.end())) {
+        h->adj_valid = false;
+        std::vector<AdjEntry> entries;
+        adjoint_entries(h->gates.data(), h->gates.size(), param_slots, entries);
+        h->adj_prog = Program();
+        build_adjoint_program(entries, h->n_int, h->opt, h->adj_prog);
+        SVMI_TRY(h->adj_dprog.upload(h->adj_prog, h->stream));
+        SVMI_TRY(h->adj_gsum.ensure(std::max<size_t>(h->adj_prog.n_grad, 1) * sizeof(double)));
+        h->adj_slots.assign(param_slots, param_slots + n_slots);
+        h->adj_n_params = n_params;
+        h->adj_valid = true;
+    }
+    const Program& bp = h->adj_prog;
+    SVMI_TRY(exec_program(h, bp, h->adj_dprog, phi, lam, h->adj_gsum.as<double>()));
     std::vector<double> g(bp.n_grad);
-    if (st == SVMI_OK && bp.n_grad) {
-        hipError_t e = hipMemcpyAsync(g.data(), gsum.p, bp.n_grad * sizeof(double), hipMemcpyDeviceToHost, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-        if (e != hipSuccess) st = fail(SVMI_ERR_DEVICE, std::string("adjoint: ") + hipGetErrorString(e));
-    }
-    if (st == SVMI_OK) {
-        (void)hipStreamSynchronize(h->stream);
-        for (uint32_t i = 0; i < bp.n_grad; i++) grad_out[bp.grad_param[i]] += 2.0 * g[i];
-    }
-    dbp.release();
-    gsum.release();
-    return st;
+    if (bp.n_grad)
+        SVMI_HIP(hipMemcpyAsync(g.data(), h->adj_gsum.p, bp.n_grad * sizeof(double), hipMemcpyDeviceToHost,
+                                h->stream));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    for (uint32_t i = 0; i < bp.n_grad; i++) grad_out[bp.grad_param[i]] += 2.0 * g[i];
+    return SVMI_OK;
 }
 
 // Batched expectation + adjoint gradients: every circuit is planned on the
@@ -1092,6 +1105,7 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
     case SVMI_OPT_PROFILE: h->profile = value != 0; return SVMI_OK;
     default: return fail(SVMI_ERR_ARG, "unknown option " + std::to_string(option));
     }
+    h->adj_valid = false;
     if (h->uploaded) {
         plan_forward(h->gates.data(), h->gates.size(), h->n_int, h->opt, h->prog);
         SVMI_TRY(use_device(h));
