@@ -157,6 +157,19 @@ def bench_batch(count=512):
             "speedup": t_loop / t_batch, "max_abs_diff": max(diff, float(np.abs(ev - ev2).max()))}
 
 
+def measured_traffic(n):
+    """HBM bytes per sweep launch of the default circuit, from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (scaled by a copy of known size);
+    None for other sizes."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    key = f"u3_brickwall n={n} layers=20"
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f).get(key)
+    return None if rec is None else rec["bytes_per_launch"]
+
+
 def stream_copy_gbps(gib=4.0, reps=5):
     """Measured device copy bandwidth (read + write bytes / time) of a
     `gib`-GiB FP64 buffer, HIP events on the current stream, best of reps."""
@@ -284,7 +297,7 @@ def main():
                    "sweep_GBps": r["passes"] * bytes_per_sweep / (r["dt"] / a.steps) / 1e9,
                    "sampling_shots_per_s": 4096 / r["sample_s"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.qubits)},
         "cpu_baseline": cpu,
         "extra": extra,
     }

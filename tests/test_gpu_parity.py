@@ -236,6 +236,29 @@ def test_adjoint_vs_oracle_parameter_shift(n, layers):
     assert np.abs(g - O.param_shift_grad(n, gates, slots, P, terms)).max() < TOL
 
 
+def test_adjoint_reuses_and_replans_backward_program():
+    # one engine: repeated call, a different parameter map, then new angles
+    n = 9
+    rng = np.random.default_rng(77)
+    gates, slots, P = W.hea(n, 3, rng)
+    terms = W.ising_terms(n, rng)
+    shared = slots.copy()
+    shared[shared >= 0] %= 5          # 5 parameters, each feeding several slots
+    gates2, _, _ = W.hea(n, 3, rng)   # same structure, new angles
+    with svmi.Engine(n) as e:
+        e.upload(gates)
+        for g_, s_, p_ in ((gates, slots, P), (gates, slots, P), (gates, shared, 5)):
+            e.reset_state()
+            E, g = e.adjoint(terms, s_, p_)
+            assert abs(E - O.expval(n, *O.run(n, g_), terms)) < TOL
+            assert np.abs(g - O.param_shift_grad(n, g_, s_, p_, terms)).max() < TOL
+        e.upload(gates2)
+        e.reset_state()
+        E, g = e.adjoint(terms, slots, P)
+        assert abs(E - O.expval(n, *O.run(n, gates2), terms)) < TOL
+        assert np.abs(g - O.param_shift_grad(n, gates2, slots, P, terms)).max() < TOL
+
+
 def test_adjoint_hea_n20_spot_check():
     gates, slots, P, terms = W.hea_ising()
     n = 20
