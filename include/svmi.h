@@ -91,10 +91,12 @@ typedef struct svmi_stats {
 /* Engine options (svmi_set_option). */
 typedef enum svmi_option {
     SVMI_OPT_FUSION = 0,      /* 1 = tiled multi-gate sweeps (default), 0 = one pass per op */
-    SVMI_OPT_TILE_QUBITS = 1, /* qubits per LDS tile (6..12, default 12)                    */
+    SVMI_OPT_TILE_QUBITS = 1, /* qubits per LDS tile (4..13, default 12)                    */
     SVMI_OPT_LOW_QUBITS = 2,  /* low qubits every tile keeps for coalescing (2..6, default 4) */
     SVMI_OPT_SWEEP_FLOPS = 3, /* per-amplitude FMA budget of one sweep (0 = unlimited)      */
-    SVMI_OPT_PROFILE = 4      /* 1 = time every launch with HIP events (svmi_stats)         */
+    SVMI_OPT_PROFILE = 4,     /* 1 = time every launch with HIP events (svmi_stats)         */
+    SVMI_OPT_PIPELINE = 5     /* 1 = persistent sweeps prefetching the next tile (default),
+                                 2 = the same with one 512-VGPR workgroup per CU, 0 = off   */
 } svmi_option;
 
 /* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,

@@ -42,6 +42,11 @@ svmi_status svmi_debug_emulate_batch(const svmi_batch_item* items, size_t n_item
 svmi_status svmi_debug_plan_batch(const svmi_batch_item* items, size_t n_items, uint64_t* n_levels,
                                   uint64_t* n_launches, uint64_t* n_tasks, uint64_t* state_bytes);
 
+/* The same for the backward (adjoint) program svmi_adjoint builds. */
+svmi_status svmi_debug_plan_adjoint(int n, const svmi_gate* gates, size_t n_gates, const int32_t* param_slots,
+                                    size_t n_params, int tile_qubits, int low_qubits, uint64_t* n_passes,
+                                    char* buf, size_t buflen);
+
 /* Planner summary: number of full-state passes and a text description
  * (truncated to buflen-1 bytes; buf may be NULL). */
 svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates,

@@ -544,6 +544,7 @@ struct svmi_sim {
     DeviceProgram dprog;
     PlanOptions opt;
     bool profile = false;
+    int pipeline = 1;
     std::vector<hipEvent_t> prof_events;
     svmi_stats stats{};
     // backward program of the uploaded circuit for the last parameter map
@@ -564,6 +565,7 @@ svmi_status ensure_kernels()
     static bool ready = false;
     if (!ready) {
         SVMI_HIP(init_sweep_kernels());
+        SVMI_HIP(init_sample_kernels());
         ready = true;
     }
     return SVMI_OK;
@@ -598,7 +600,8 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
                 partials = h->scratch[5].as<double>();
             }
             SVMI_HIP(launch_sweep(h->stream, prog.n_states, s0, s1, sw, dp.phases.as<PhaseInfo>(),
-                                  dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials));
+                                  dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials,
+                                  h->pipeline));
             if (partials)
                 SVMI_HIP(launch_sum_rows(h->stream, partials, sw.n_grad, sw.n_tiles, grad_sums + sw.grad_base));
         } else {
@@ -1091,7 +1094,7 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
     switch (option) {
     case SVMI_OPT_FUSION: h->opt.fusion = value != 0; break;
     case SVMI_OPT_TILE_QUBITS:
-        if (value < kRegBits || value > kMaxTileQubits) return fail(SVMI_ERR_ARG, "tile qubits must be in [4, 12]");
+        if (value < kRegBits || value > kMaxTileQubits) return fail(SVMI_ERR_ARG, "tile qubits must be in [4, 13]");
         h->opt.tile_qubits = int(value);
         break;
     case SVMI_OPT_LOW_QUBITS:
@@ -1103,6 +1106,10 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
         h->opt.sweep_budget = long(value);
         break;
     case SVMI_OPT_PROFILE: h->profile = value != 0; return SVMI_OK;
+    case SVMI_OPT_PIPELINE:
+        if (value < 0 || value > 2) return fail(SVMI_ERR_ARG, "pipeline must be 0, 1 or 2");
+        h->pipeline = int(value);
+        return SVMI_OK;
     default: return fail(SVMI_ERR_ARG, "unknown option " + std::to_string(option));
     }
     h->adj_valid = false;
@@ -1438,6 +1445,28 @@ svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates, int f
     Program prog;
     plan_forward(gates, n_gates, std::max(n, kRegBits), debug_options(fusion, tile_qubits, low_qubits, sweep_budget),
                  prog);
+    if (n_passes) *n_passes = prog.steps.size();
+    if (buf && buflen) {
+        const std::string s = describe_program(prog);
+        const size_t m = std::min(buflen - 1, s.size());
+        std::memcpy(buf, s.data(), m);
+        buf[m] = 0;
+    }
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_plan_adjoint(int n, const svmi_gate* gates, size_t n_gates, const int32_t* param_slots,
+                                    size_t n_params, int tile_qubits, int low_qubits, uint64_t* n_passes, char* buf,
+                                    size_t buflen)
+{
+    if (n < 1 || n > kMaxQubits) return fail(SVMI_ERR_ARG, "bad n");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    SVMI_TRY(check_gates(gates, n_gates, n));
+    SVMI_TRY(check_params(gates, n_gates, param_slots, n_params));
+    std::vector<AdjEntry> entries;
+    adjoint_entries(gates, n_gates, param_slots, entries);
+    Program prog;
+    build_adjoint_program(entries, std::max(n, kRegBits), debug_options(1, tile_qubits, low_qubits, 0), prog);
     if (n_passes) *n_passes = prog.steps.size();
     if (buf && buflen) {
         const std::string s = describe_program(prog);

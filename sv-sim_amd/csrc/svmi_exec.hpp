@@ -173,6 +173,20 @@ SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[kRegs], const c64* 
         pr = on ? tr : pr;
         pi = on ? ti : pi;
     }
+    // tables over bits outside the tile: index and factor are tile-uniform
+    const c64* tab = coef + h.coef + 2 * cnt;
+    for (uint32_t c = 0; c < h.s1; c++) {
+        const PhaseChunk* hc = reinterpret_cast<const PhaseChunk*>(tab);
+        const uint32_t nb = hc->nbits;
+        uint32_t idx = 0;
+        for (uint32_t i = 0; i < nb; i++) idx |= uint32_t((base >> hc->pos[i]) & 1u) << i;
+        const c64 f = tab[1 + idx];
+        const double tr = SVMI_FMA(pr, f.re, -(pi * f.im));
+        const double ti = SVMI_FMA(pr, f.im, pi * f.re);
+        pr = tr;
+        pi = ti;
+        tab += 1 + (1u << nb);
+    }
     const c64 ph{pr, pi};
     const uint32_t rsel = h.rsel;
 #pragma unroll

@@ -55,9 +55,10 @@ SVMI_HD inline c64 cconjmul(c64 a, c64 b) { return c64{a.re * b.re + a.im * b.im
 
 constexpr int kRegBits = 4;                 // register bits per phase
 constexpr int kRegs = 1 << kRegBits;        // amplitudes per thread per group
-constexpr int kMaxTileQubits = 12;          // LDS tile: 2^12 * 16 B = 64 KiB
+constexpr int kMaxTileQubits = 13;          // LDS tile: 2^13 * 16 B = 128 KiB
 constexpr int kMaxTileQubitsAdj = 11;       // two states (phi, lambda): 2 * 32 KiB
-constexpr int kThreads = 256;               // threads per tile workgroup
+constexpr int kThreads = 256;               // threads of the reduction / Pauli workgroups
+constexpr int kMaxTileThreads = 1 << (kMaxTileQubits - kRegBits);   // 512
 constexpr int kMaxDenseQubits = 5;          // standalone dense ops
 constexpr int kMaxGradsPerSweep = 256;      // adjoint accumulators per sweep (LDS)
 
@@ -86,6 +87,17 @@ struct PhaseEntry {
     double ph_re, ph_im;
 };
 static_assert(sizeof(PhaseEntry) == 32, "PhaseEntry layout");
+
+// Phase-product table header (one coefficient slot), followed by 2^nbits
+// products: entry idx = product of the phases of the bits set in idx, bit i
+// of idx <-> global index bit pos[i].
+constexpr int kPhaseChunkBits = 8;
+struct PhaseChunk {
+    uint8_t pos[kPhaseChunkBits];
+    uint32_t nbits;
+    uint32_t pad;
+};
+static_assert(sizeof(PhaseChunk) == 16, "PhaseChunk layout");
 
 // Which state(s) an op applies to in adjoint sweeps.
 enum : uint8_t { DOP_TO_PSI = 1, DOP_TO_LAM = 2, DOP_TO_BOTH = 3 };

@@ -19,47 +19,58 @@ namespace svmi {
 
 // ------------------------------------------------------------------ sweeps
 
-// One tile of one sweep.  Threads tid < nt (= 2^(k-4)) own one register
-// group per phase; with kIdle (batched kernel: blockDim >= nt) the others
-// only join the barriers.  Without kIdle every thread is active, which keeps
-// the phase body free of divergence (and its register groups out of
-// scratch).  init_zero: the tile is generated as |0..0> instead of loaded.
-template <int NS, bool kIdle>
-__device__ __attribute__((always_inline)) inline void
-sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
-           const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
-           const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
-           double* __restrict__ grad_partials, uint64_t tile, uint32_t nt, unsigned char* smem,
-           bool init_zero)
-{
-    const uint32_t k = sw.k;
-    const uint32_t tsize = 1u << k;
-    c64* t0 = reinterpret_cast<c64*>(smem);
-    c64* t1 = t0 + (NS == 2 ? tsize : 0);
-    double* gacc = reinterpret_cast<double*>(t0 + tsize * NS);
+// Tile staging.  A tile is 2^k amplitudes; thread tid owns tile-local
+// indices l = tid + nt*j (j < 16); global index = base | row(l >> c) | (l & cmask).
+// LDS holds the tile XOR-swizzled (lds_slot).
 
-    const uint32_t tid = threadIdx.x;
-    const bool active = !kIdle || tid < nt;
-    const uint32_t nwaves = (nt + 63) / 64;
+// HBM (or |0..0> when init_zero) -> registers r[j] for tile `tile`.
+__device__ __attribute__((always_inline)) inline void
+tile_fetch(c64 (&r)[kRegs], const c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht,
+           uint64_t tile, uint32_t tid, uint32_t nt, bool init_zero)
+{
     const uint64_t base = tile_base(sw, tile);
     const uint32_t cmask = (1u << sw.c) - 1u;
-    const uint64_t* ht = hitab + sw.hitab;
-
-    if (active) {
 #pragma unroll
-        for (int j = 0; j < kRegs; j++) {
-            const uint32_t l = tid + nt * j;
-            const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
-            t0[lds_slot(l)] = init_zero ? c64{g == 0 ? 1.0 : 0.0, 0.0} : s0[g];
-            if (NS == 2) t1[lds_slot(l)] = s1[g];
-        }
+    for (int j = 0; j < kRegs; j++) {
+        const uint32_t l = tid + nt * j;
+        const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
+        r[j] = init_zero ? c64{g == 0 ? 1.0 : 0.0, 0.0} : s[g];
     }
-    if (NS == 2)
-        for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
-    __syncthreads();
+}
 
-    // every thread runs the phase loop (uniform barriers); only threads that
-    // own a register group do work
+__device__ __attribute__((always_inline)) inline void
+tile_put(const c64 (&r)[kRegs], c64* t, uint32_t tid, uint32_t nt)
+{
+#pragma unroll
+    for (int j = 0; j < kRegs; j++) t[lds_slot(tid + nt * j)] = r[j];
+}
+
+// LDS -> HBM for tile `tile`.
+__device__ __attribute__((always_inline)) inline void
+tile_store(const c64* t, c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht, uint64_t tile,
+           uint32_t tid, uint32_t nt)
+{
+    const uint64_t base = tile_base(sw, tile);
+    const uint32_t cmask = (1u << sw.c) - 1u;
+#pragma unroll
+    for (int j = 0; j < kRegs; j++) {
+        const uint32_t l = tid + nt * j;
+        s[base | ht[l >> sw.c] | (l & cmask)] = t[lds_slot(l)];
+    }
+}
+
+// The phases of one sweep on the tile in LDS (t0, and t1 for NS = 2).
+// Threads tid < nt own one register group per phase; with kIdle (batched
+// kernel: blockDim >= nt) the others only join the barriers.  Without kIdle
+// every thread is active, which keeps the phase body free of divergence
+// (and its register groups out of scratch).  Ends with a barrier.
+template <int NS, bool kIdle>
+__device__ __attribute__((always_inline)) inline void
+tile_phases(c64* t0, c64* t1, double* gacc, const SweepInfo& sw, const PhaseInfo* __restrict__ phases,
+            const DOp* __restrict__ ops, const c64* __restrict__ coef, uint64_t base, uint32_t tid, uint32_t nt)
+{
+    const bool active = !kIdle || tid < nt;
+    const uint32_t nwaves = (nt + 63) / 64;
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
         if (active) {
             const PhaseInfo ph = phases[p];
@@ -98,15 +109,43 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
         }
         __syncthreads();
     }
+}
+
+// One tile of one sweep: load, phases, store (and for NS = 2 the tile's
+// gradient partials).  init_zero: the tile is generated as |0..0>.
+template <int NS, bool kIdle>
+__device__ __attribute__((always_inline)) inline void
+sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
+           const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
+           const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
+           double* __restrict__ grad_partials, uint64_t tile, uint32_t nt, unsigned char* smem,
+           bool init_zero)
+{
+    const uint32_t tsize = 1u << sw.k;
+    c64* t0 = reinterpret_cast<c64*>(smem);
+    c64* t1 = t0 + (NS == 2 ? tsize : 0);
+    double* gacc = reinterpret_cast<double*>(t0 + tsize * NS);
+    const uint32_t tid = threadIdx.x;
+    const bool active = !kIdle || tid < nt;
+    const uint32_t nwaves = (nt + 63) / 64;
+    const uint64_t* ht = hitab + sw.hitab;
 
     if (active) {
-#pragma unroll
-        for (int j = 0; j < kRegs; j++) {
-            const uint32_t l = tid + nt * j;
-            const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
-            s0[g] = t0[lds_slot(l)];
-            if (NS == 2) s1[g] = t1[lds_slot(l)];
+        c64 r[kRegs];
+        tile_fetch(r, s0, sw, ht, tile, tid, nt, init_zero);
+        tile_put(r, t0, tid, nt);
+        if (NS == 2) {
+            tile_fetch(r, s1, sw, ht, tile, tid, nt, false);
+            tile_put(r, t1, tid, nt);
         }
+    }
+    if (NS == 2)
+        for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
+    __syncthreads();
+    tile_phases<NS, kIdle>(t0, t1, gacc, sw, phases, ops, coef, tile_base(sw, tile), tid, nt);
+    if (active) {
+        tile_store(t0, s0, sw, ht, tile, tid, nt);
+        if (NS == 2) tile_store(t1, s1, sw, ht, tile, tid, nt);
     }
     // gradient partials are stored grad-major ([i][tile]) so the sum over
     // tiles of one gradient reads one contiguous segment
@@ -119,8 +158,9 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     }
 }
 
-// One state: 64 KiB tile -> 2 workgroups (8 waves) per CU, so 256 VGPRs.
-__global__ void __launch_bounds__(kThreads, 2)
+// One state, one tile per workgroup: up to 8 waves per CU (one 2^13 tile of
+// 512 threads, or two 2^12 tiles of 256), so 256 VGPRs per thread.
+__global__ void __launch_bounds__(kMaxTileThreads, 1)
 k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
          uint64_t tile_offset)
@@ -128,6 +168,40 @@ k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__
     extern __shared__ __align__(16) unsigned char smem[];
     sweep_tile<1, false>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x,
                          smem, false);
+}
+
+// One state, persistent and software-pipelined: workgroup w handles tiles
+// w, w + gridDim.x, ...  While the phases of tile t run out of LDS, the
+// loads of the next tile are already in flight into registers, so HBM
+// traffic overlaps the arithmetic inside each workgroup (co-resident
+// workgroups start in step and would otherwise load and compute together).
+// kWide: 256-thread workgroups with the full 512-VGPR budget (one per CU);
+// otherwise up to 512 threads and 256 VGPRs.
+template <bool kWide>
+__global__ void __launch_bounds__(kWide ? 256 : kMaxTileThreads, 1)
+k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
+          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
+          uint64_t tile_offset, uint64_t n_tiles)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    c64* t0 = reinterpret_cast<c64*>(smem);
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
+    const uint64_t* ht = hitab + sw.hitab;
+    uint64_t t = blockIdx.x;
+    c64 nx[kRegs];
+    tile_fetch(nx, s0, sw, ht, tile_offset + t, tid, nt, false);
+    for (;;) {
+        tile_put(nx, t0, tid, nt);
+        __syncthreads();
+        const uint64_t tn = t + gridDim.x;
+        const bool more = tn < n_tiles;
+        if (more) tile_fetch(nx, s0, sw, ht, tile_offset + tn, tid, nt, false);
+        tile_phases<1, false>(t0, nullptr, nullptr, sw, phases, ops, coef, tile_base(sw, tile_offset + t), tid, nt);
+        tile_store(t0, s0, sw, ht, tile_offset + t, tid, nt);
+        if (!more) break;
+        __syncthreads();  // the store's LDS reads finish before the next put
+        t = tn;
+    }
 }
 
 // Two states (adjoint): one wave per SIMD, registers for both groups.
@@ -175,7 +249,7 @@ __device__ inline uint32_t batch_step_of(const BatchStep* __restrict__ steps, ui
 
 // kInit: the steps are first forward sweeps (tiles generated as |0..0>).
 template <bool kInit>
-__global__ void __launch_bounds__(kThreads, 2)
+__global__ void __launch_bounds__(kMaxTileThreads, 1)
 k_batch_fwd(SVMI_BATCH_TABLES, uint32_t s0, uint32_t s1, uint64_t task0)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -288,13 +362,45 @@ size_t sweep_smem_bytes(const SweepInfo& sw, int ns)
     return (size_t(16) << sw.k) * ns + size_t(8) * sw.n_grad * nwaves;
 }
 
+// Workgroups of k_sweep1p resident at once for tile size k (occupancy x CUs).
+template <bool kWide>
+static uint64_t pipeline_grid(uint32_t k)
+{
+    static uint64_t cache[kMaxTileQubits + 1] = {};
+    if (cache[k]) return cache[k];
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_sweep1p<kWide>),
+                                                     1 << (k - kRegBits), size_t(16) << k) != hipSuccess ||
+        per < 1)
+        per = 1;
+    (void)hipGetLastError();
+    cache[k] = uint64_t(cus) * uint64_t(per);
+    return cache[k];
+}
+
 hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInfo& sw,
                         const PhaseInfo* phases, const DOp* ops, const c64* coef,
-                        const uint64_t* hitab, double* grad_partials)
+                        const uint64_t* hitab, double* grad_partials, int pipeline)
 {
     const uint32_t nt = 1u << (sw.k - kRegBits);
     const size_t smem = sweep_smem_bytes(sw, ns);
     if (smem > g_max_smem) return hipErrorInvalidConfiguration;
+    if (ns == 1 && pipeline) {
+        const bool wide = pipeline == 2 && nt <= 256;
+        const uint64_t resident = wide ? pipeline_grid<true>(sw.k) : pipeline_grid<false>(sw.k);
+        if (sw.n_tiles > resident) {
+            if (wide)
+                hipLaunchKernelGGL(k_sweep1p<true>, dim3(uint32_t(resident)), dim3(nt), smem, st, s0, sw, phases, ops,
+                                   coef, hitab, uint64_t(0), sw.n_tiles);
+            else
+                hipLaunchKernelGGL(k_sweep1p<false>, dim3(uint32_t(resident)), dim3(nt), smem, st, s0, sw, phases,
+                                   ops, coef, hitab, uint64_t(0), sw.n_tiles);
+            return hipGetLastError();
+        }
+    }
     const uint64_t max_grid = 1ull << 30;
     for (uint64_t t0 = 0; t0 < sw.n_tiles; t0 += max_grid) {
         const uint64_t cnt = sw.n_tiles - t0 < max_grid ? sw.n_tiles - t0 : max_grid;
@@ -325,6 +431,12 @@ hipError_t init_sweep_kernels()
     if (want > 64 * 1024) {
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1p<false>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1p<true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e == hipSuccess)
             e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep2),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
@@ -602,6 +714,58 @@ k_block_totals(const c64* __restrict__ s, uint32_t chunk, double* __restrict__ t
     if (threadIdx.x == blockDim.x - 1) totals[b] = acc;
 }
 
+// The same two passes with the block's probabilities staged in LDS first:
+// loaded lane-consecutive (coalesced), stored so that thread t's chunk sits
+// at t * (chunk + 1) (the pad spreads the per-thread runs over the banks).
+// Every sum keeps the order of the direct kernels above, so the totals and
+// samples are bit-identical to theirs.
+__device__ inline uint32_t staged_index(uint32_t j, uint32_t lg) { return j + (j >> lg); }
+
+__device__ void stage_probs(const c64* __restrict__ blk, uint32_t lg, double* lp)
+{
+    const uint32_t bs = blockDim.x << lg;
+    for (uint32_t j = threadIdx.x; j < bs; j += blockDim.x) lp[staged_index(j, lg)] = prob_rn(blk[j]);
+    __syncthreads();
+}
+
+__device__ double thread_prefix_staged(const double* lp, uint32_t lg, double* lds_w)
+{
+    const uint32_t t = threadIdx.x, nthr = blockDim.x, chunk = 1u << lg;
+    const double* mine = lp + t * (chunk + 1);
+    double T = 0.0;
+    for (uint32_t i = 0; i < chunk; i++) T = __dadd_rn(T, mine[i]);
+    const int width = nthr < 64 ? int(nthr) : 64;
+    const int lane = int(t & 63u);
+    double inc = T;
+    for (int d = 1; d < width; d <<= 1) {
+        const double u = __shfl_up(inc, d, width);
+        if (lane >= d) inc = __dadd_rn(inc, u);
+    }
+    double exc = __shfl_up(inc, 1, width);
+    if (lane == 0) exc = 0.0;
+    const uint32_t w = t >> 6;
+    if (lane == width - 1) lds_w[w] = inc;
+    __syncthreads();
+    double wpre = 0.0;
+    for (uint32_t v = 0; v < w; v++) wpre = __dadd_rn(wpre, lds_w[v]);
+    __syncthreads();
+    return w == 0 ? exc : __dadd_rn(wpre, exc);
+}
+
+__global__ void __launch_bounds__(256)
+k_block_totals_staged(const c64* __restrict__ s, uint32_t lg, double* __restrict__ totals)
+{
+    extern __shared__ double lp[];
+    __shared__ double lds_w[4];
+    const uint64_t b = blockIdx.x;
+    const uint32_t chunk = 1u << lg;
+    stage_probs(s + (b * blockDim.x << lg), lg, lp);
+    double acc = thread_prefix_staged(lp, lg, lds_w);
+    const double* mine = lp + threadIdx.x * (chunk + 1);
+    for (uint32_t i = 0; i < chunk; i++) acc = __dadd_rn(acc, mine[i]);
+    if (threadIdx.x == blockDim.x - 1) totals[b] = acc;
+}
+
 // One workgroup per (block, shot range).  job = {block, first shot, n shots}.
 __global__ void __launch_bounds__(256)
 k_sample(const c64* __restrict__ s, uint32_t chunk, const uint64_t* __restrict__ jobs,
@@ -637,13 +801,79 @@ k_sample(const c64* __restrict__ s, uint32_t chunk, const uint64_t* __restrict__
     }
 }
 
+__global__ void __launch_bounds__(256)
+k_sample_staged(const c64* __restrict__ s, uint32_t lg, const uint64_t* __restrict__ jobs,
+                const double* __restrict__ offsets, const double* __restrict__ draws,
+                uint64_t* __restrict__ out)
+{
+    extern __shared__ double lp[];
+    __shared__ double lds_w[4];
+    __shared__ unsigned long long best;
+    const uint32_t chunk = 1u << lg;
+    const uint64_t b = jobs[3 * blockIdx.x + 0];
+    const uint64_t first = jobs[3 * blockIdx.x + 1];
+    const uint64_t cnt = jobs[3 * blockIdx.x + 2];
+    const uint64_t bsize = uint64_t(blockDim.x) << lg;
+    stage_probs(s + b * bsize, lg, lp);
+    const double P = thread_prefix_staged(lp, lg, lds_w);
+    const double O = offsets[blockIdx.x];
+    const double* mine = lp + threadIdx.x * (chunk + 1);
+    for (uint64_t q = 0; q < cnt; q++) {
+        const double r = draws[first + q];
+        if (threadIdx.x == 0) best = ~0ull;
+        __syncthreads();
+        double acc = P;
+        uint64_t found = ~0ull;
+        for (uint32_t i = 0; i < chunk; i++) {
+            acc = __dadd_rn(acc, mine[i]);
+            if (__dadd_rn(O, acc) > r) {
+                found = uint64_t(threadIdx.x) * chunk + i;
+                break;
+            }
+        }
+        if (found != ~0ull) atomicMin(&best, (unsigned long long)found);
+        __syncthreads();
+        if (threadIdx.x == 0) out[first + q] = best == ~0ull ? 0 : b * bsize + best;
+        __syncthreads();
+    }
+}
+
+static size_t staged_smem(uint32_t nthr, uint32_t chunk) { return (size_t(nthr) * chunk + nthr) * sizeof(double); }
+static size_t g_staged_smem = 0;   // dynamic LDS the staged sampling kernels may use
+
+hipError_t init_sample_kernels()
+{
+    const size_t want = sweep_smem_limit();
+    g_staged_smem = 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_totals_staged),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sample_staged),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) == hipSuccess)
+        g_staged_smem = want;
+    (void)hipGetLastError();
+    return hipSuccess;
+}
+
+static uint32_t log2_u32(uint32_t v)
+{
+    uint32_t l = 0;
+    while ((1u << l) < v) l++;
+    return l;
+}
+
 hipError_t launch_block_totals(hipStream_t st, const c64* s, uint64_t nblocks, uint32_t nthr,
                                uint32_t chunk, double* totals)
 {
+    const size_t smem = staged_smem(nthr, chunk);
+    const bool staged = smem <= g_staged_smem;
     for (uint64_t b0 = 0; b0 < nblocks; b0 += (1ull << 30)) {
         const uint64_t cnt = nblocks - b0 < (1ull << 30) ? nblocks - b0 : (1ull << 30);
-        hipLaunchKernelGGL(k_block_totals, dim3(uint32_t(cnt)), dim3(nthr), 0, st,
-                           s + b0 * uint64_t(nthr) * chunk, chunk, totals + b0);
+        if (staged)
+            hipLaunchKernelGGL(k_block_totals_staged, dim3(uint32_t(cnt)), dim3(nthr), smem, st,
+                               s + b0 * uint64_t(nthr) * chunk, log2_u32(chunk), totals + b0);
+        else
+            hipLaunchKernelGGL(k_block_totals, dim3(uint32_t(cnt)), dim3(nthr), 0, st,
+                               s + b0 * uint64_t(nthr) * chunk, chunk, totals + b0);
     }
     return hipGetLastError();
 }
@@ -653,7 +883,12 @@ hipError_t launch_sample(hipStream_t st, const c64* s, uint32_t nthr, uint32_t c
                          const double* draws, uint64_t* out)
 {
     if (njobs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sample, dim3(njobs), dim3(nthr), 0, st, s, chunk, jobs, offsets, draws, out);
+    const size_t smem = staged_smem(nthr, chunk);
+    if (smem <= g_staged_smem)
+        hipLaunchKernelGGL(k_sample_staged, dim3(njobs), dim3(nthr), smem, st, s, log2_u32(chunk), jobs, offsets,
+                           draws, out);
+    else
+        hipLaunchKernelGGL(k_sample, dim3(njobs), dim3(nthr), 0, st, s, chunk, jobs, offsets, draws, out);
     return hipGetLastError();
 }
 

@@ -70,11 +70,14 @@ size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad);
 hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L);
 
 hipError_t init_sweep_kernels();
+hipError_t init_sample_kernels();   // after init_sweep_kernels
 size_t sweep_smem_limit();
 size_t sweep_smem_bytes(const SweepInfo& sw, int ns);
+// pipeline != 0: one-state sweeps larger than one resident grid run
+// persistent and prefetch the next tile (k_sweep1p); 2 = 512-VGPR variant
 hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInfo& sw,
                         const PhaseInfo* phases, const DOp* ops, const c64* coef,
-                        const uint64_t* hitab, double* grad_partials);
+                        const uint64_t* hitab, double* grad_partials, int pipeline);
 hipError_t launch_standalone(hipStream_t st, c64* s, int n, const StandaloneOp& so, const c64* coef);
 hipError_t launch_fill_basis(hipStream_t st, c64* s, uint64_t dim);
 hipError_t launch_gather(hipStream_t st, const c64* s, uint64_t dim, const uint64_t* idx, uint64_t cnt, c64* out);

@@ -29,11 +29,14 @@ uint64_t op_mask(const LOp& op)
     return m;
 }
 
+bool is_register_perm(const LOp& op);
+
 // FMA cost per amplitude (for the sweep budget).
 long op_cost(const LOp& op, bool grad)
 {
     if (op.kind == LOp::DIAG) return grad ? 6 : 4;
     if (op.kind == LOp::SRN) return 4;
+    if (!grad && is_register_perm(op)) return 1;
     return (4L << op.k) * (grad ? 2 : 1);
 }
 
@@ -95,6 +98,22 @@ bool is_perm(const LOp& op, const int* src /* out row r <- in col src[r] */)
     return true;
 }
 
+// The permutations the device runs as register moves (DOP_X, DOP_CX,
+// DOP_SWAP): local index bit i <-> op.q[i].
+bool is_register_perm(const LOp& op)
+{
+    if (op.kind != LOp::DENSE) return false;
+    if (op.k == 1) {
+        const int x[2] = {1, 0};
+        return is_perm(op, x);
+    }
+    if (op.k == 2) {
+        const int cx01[4] = {0, 3, 2, 1}, cx10[4] = {0, 1, 3, 2}, sw[4] = {0, 2, 1, 3};
+        return is_perm(op, cx01) || is_perm(op, cx10) || is_perm(op, sw);
+    }
+    return false;
+}
+
 // diag whose entries are all 1 except the all-ones index
 bool is_cphase(const SOp& s)
 {
@@ -139,19 +158,64 @@ struct Encoder {
             groups[g].second.push_back(e);
         }
         for (auto& gr : groups) {
+            // entries conditioned on one bit outside the tile are uniform per
+            // tile: they go into product tables over chunks of up to 8 such
+            // bits (one table read per chunk on the device); the others stay
+            // per-entry
+            std::vector<PhaseEntry> rest;
+            std::vector<int> ebits;
+            std::vector<c64> ebit_phase;
+            for (const PhaseEntry& e : gr.second) {
+                if (e.gmask == 0 && __builtin_popcountll(e.emask) == 1) {
+                    const int b = __builtin_ctzll(e.emask);
+                    size_t i = 0;
+                    while (i < ebits.size() && ebits[i] != b) i++;
+                    const c64 ph{e.ph_re, e.ph_im};
+                    if (i == ebits.size()) {
+                        ebits.push_back(b);
+                        ebit_phase.push_back(ph);
+                    } else {
+                        ebit_phase[i] = cmul(ebit_phase[i], ph);
+                    }
+                } else {
+                    rest.push_back(e);
+                }
+            }
+            if (ebits.size() < 2) {
+                rest = gr.second;
+                ebits.clear();
+            }
             DOp d;
             std::memset(&d, 0, sizeof(d));
             d.targets = prog.n_states == 2 ? DOP_TO_BOTH : DOP_TO_PSI;
             d.h.type = DOP_PHASESET;
             d.h.rsel = gr.first;
-            d.h.s0 = uint32_t(gr.second.size());
+            d.h.s0 = uint32_t(rest.size());
             d.h.coef = uint32_t(prog.coef.size());
-            for (const PhaseEntry& e : gr.second) {
+            for (const PhaseEntry& e : rest) {
                 c64 two[2];
                 std::memcpy(two, &e, sizeof(e));
                 prog.coef.push_back(two[0]);
                 prog.coef.push_back(two[1]);
             }
+            uint32_t chunks = 0;
+            for (size_t c0 = 0; c0 < ebits.size(); c0 += kPhaseChunkBits, chunks++) {
+                const uint32_t nb = uint32_t(std::min<size_t>(kPhaseChunkBits, ebits.size() - c0));
+                PhaseChunk hdr;
+                std::memset(&hdr, 0, sizeof(hdr));
+                for (uint32_t i = 0; i < nb; i++) hdr.pos[i] = uint8_t(ebits[c0 + i]);
+                hdr.nbits = nb;
+                c64 slot;
+                std::memcpy(&slot, &hdr, sizeof(slot));
+                prog.coef.push_back(slot);
+                for (uint32_t idx = 0; idx < (1u << nb); idx++) {
+                    c64 f{1.0, 0.0};
+                    for (uint32_t i = 0; i < nb; i++)
+                        if ((idx >> i) & 1) f = cmul(f, ebit_phase[c0 + i]);
+                    prog.coef.push_back(f);
+                }
+            }
+            d.h.s1 = chunks;
             prog.ops.push_back(d);
         }
     }
@@ -277,9 +341,13 @@ struct Encoder {
 // ---------------------------------------------------------------------------
 constexpr long kOpOverhead = 8;  // per-op decode cost in FMA per amplitude
 
+bool is_register_perm(const LOp& op);
+
+// X, CX and SWAP run as register moves; other dense ops as matrix products.
 long reg_cost(const LOp& op)
 {
     if (op.kind == LOp::DIAG) return 2;
+    if (is_register_perm(op)) return 1;
     return 4L << op.k;
 }
 
@@ -729,7 +797,10 @@ std::string describe_program(const Program& prog)
                     const DOp& d = prog.ops[o];
                     static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS"};
                     ss << " " << nm[d.h.type];
-                    if (d.h.type == DOP_PHASESET) ss << d.h.s0;
+                    if (d.h.type == DOP_PHASESET) {
+                        ss << d.h.s0;
+                        if (d.h.s1) ss << "t" << d.h.s1;
+                    }
                     if (d.h.type == DOP_DENSE || d.h.type == DOP_GRAD_DENSE) ss << __builtin_popcount(d.h.smask);
                 }
                 ss << "\n";

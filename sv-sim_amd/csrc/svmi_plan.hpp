@@ -11,7 +11,7 @@ namespace svmi {
 
 struct PlanOptions {
     bool fusion = true;        // tiled multi-op sweeps; false = one pass per op
-    int tile_qubits = 12;      // k: qubits per LDS tile
+    int tile_qubits = 12;      // k: qubits per LDS tile (13 allowed: one 128 KiB tile per CU)
     int low_qubits = 4;        // c: physical qubits 0..c-1 in every tile
     long sweep_budget = 0;     // per-amplitude FMA budget of one sweep (0 = none)
 };

@@ -33,7 +33,7 @@ PAULI_DTYPE = np.dtype([("coeff", "<f8"), ("xmask", "<u8"), ("zmask", "<u8")])
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
           5: "ERR_DEVICE", 6: "ERR_OOM", 7: "ERR_UNSUPPORTED"}
 
-OPT_FUSION, OPT_TILE_QUBITS, OPT_LOW_QUBITS, OPT_SWEEP_FLOPS, OPT_PROFILE = 0, 1, 2, 3, 4
+OPT_FUSION, OPT_TILE_QUBITS, OPT_LOW_QUBITS, OPT_SWEEP_FLOPS, OPT_PROFILE, OPT_PIPELINE = 0, 1, 2, 3, 4, 5
 
 # Every exported entry point (include/svmi.h, include/svmi_debug.h).
 EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state", "svmi_set_random_state",
@@ -41,7 +41,7 @@ EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
            "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
-           "svmi_debug_plan_batch"]
+           "svmi_debug_plan_batch", "svmi_debug_plan_adjoint"]
 
 
 class SvmiError(RuntimeError):
@@ -114,6 +114,8 @@ def lib():
         L.svmi_debug_emulate_batch.argtypes = [_vp, _sz, _vp, _vp]
         L.svmi_debug_plan.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
+        L.svmi_debug_plan_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("svmi_num_qubits", "svmi_last_error", "svmi_abi_version"):
@@ -347,6 +349,17 @@ def emulate_adjoint(n, gates, terms, param_slots, n_params, state, tile_qubits=1
 def emulate_batch(items):
     """Host execution of batch_adjoint's task queue (planner check)."""
     return batch_adjoint(items, _emulate=True)
+
+
+def plan_adjoint(n, gates, param_slots, n_params, tile_qubits=11, low_qubits=4):
+    """(passes, description) of the backward program of svmi_adjoint."""
+    g = as_gates(gates)
+    ps = np.ascontiguousarray(param_slots, dtype=np.int32).reshape(-1)
+    buf = ctypes.create_string_buffer(1 << 20)
+    passes = ctypes.c_uint64(0)
+    _check(lib().svmi_debug_plan_adjoint(int(n), _ptr(g), len(g), _ptr(ps), int(n_params), int(tile_qubits),
+                                         int(low_qubits), ctypes.byref(passes), buf, len(buf)))
+    return passes.value, buf.value.decode()
 
 
 def plan(n, gates, fusion=True, tile_qubits=12, low_qubits=4, sweep_budget=0):

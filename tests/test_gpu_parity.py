@@ -79,7 +79,7 @@ def test_gate_kats_all_ops():
 
 
 @pytest.mark.parametrize("n", [1, 3, 4, 6, 9, 13, 15])
-@pytest.mark.parametrize("opts", [{}, {"fusion": 0}, {"tile": 6, "low": 2}, {"tile": 8, "low": 0}, {"tile": 11}])
+@pytest.mark.parametrize("opts", [{}, {"fusion": 0}, {"tile": 6, "low": 2}, {"tile": 8, "low": 0}, {"tile": 11}, {"tile": 12}])
 def test_random_all_op_circuits(n, opts):
     rng = np.random.default_rng(500 + n)
     gates = []

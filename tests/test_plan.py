@@ -28,8 +28,8 @@ def _err(a, b):
     return max(np.abs(a[0] - b[0]).max(), np.abs(a[1] - b[1]).max())
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 11, 13])
-@pytest.mark.parametrize("fusion,tile,low", [(0, 12, 4), (1, 12, 4), (1, 6, 2), (1, 4, 0), (1, 9, 3)])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 11, 13, 15])
+@pytest.mark.parametrize("fusion,tile,low", [(0, 13, 4), (1, 13, 4), (1, 12, 4), (1, 6, 2), (1, 4, 0), (1, 9, 3)])
 def test_emulated_program_matches_oracle(n, fusion, tile, low):
     rng = np.random.default_rng(100 + n)
     gates = _all_ops_circuit(n, 3, rng)
@@ -141,3 +141,15 @@ def test_batch_rejects_five_qubit_gates():
     with pytest.raises(svmi.SvmiError) as e:
         svmi.emulate_batch([item])
     assert e.value.status == 7
+
+
+@pytest.mark.parametrize("n,tile,low", [(16, 6, 2), (18, 8, 4), (17, 12, 4)])
+def test_phase_product_tables_match_oracle(n, tile, low):
+    # QFT-pattern circuits put many controlled phases on qubits outside the
+    # tile; those become per-tile product tables (PS..t..) -- check the plan
+    # really uses them and that the result is unchanged
+    gates = W.qft(n=n)
+    _, desc = svmi.plan(n, gates, tile_qubits=tile, low_qubits=low)
+    assert "t" in "".join(tok for tok in desc.split() if tok.startswith("PS"))
+    st = W.random_state(n, seed=n + 40)
+    assert _err(svmi.emulate(n, gates, st, tile_qubits=tile, low_qubits=low), O.run(n, gates, st)) < 1e-12
