@@ -252,14 +252,15 @@ svmi_status verify_batch(const BatchPlan& bp)
         if (L.step_begin != next || L.step_end <= L.step_begin || L.step_end > bp.steps.size() ||
             level_of[L.step_begin] != level_of[L.step_end - 1] ||
             L.task_begin != bp.steps[L.step_begin].task_begin ||
-            batch_launch_smem(L.kind, L.k, L.max_grad) > size_t(160) * 1024)
+            batch_launch_smem(L.kind, L.k, L.max_grad, L.max_xf) > size_t(160) * 1024)
             return fail(SVMI_ERR_STATE, "batch plan: bad launch");
         uint64_t nt = 0;
         for (uint32_t s = L.step_begin; s < L.step_end; s++) {
             const BatchStep& b = bp.steps[s];
             const bool sweep = b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ;
             if (b.kind != L.kind || (sweep && bp.sweeps[b.sweep].k != L.k) || b.init_zero != L.init ||
-                (b.kind == BSTEP_ADJ && bp.sweeps[b.sweep].n_grad > L.max_grad))
+                (b.kind == BSTEP_ADJ && bp.sweeps[b.sweep].n_grad > L.max_grad) ||
+                (sweep && bp.sweeps[b.sweep].n_xf > L.max_xf))
                 return fail(SVMI_ERR_STATE, "batch plan: mixed launch");
             nt += b.n_tasks;
         }
@@ -287,9 +288,15 @@ svmi_status verify_batch(const BatchPlan& bp)
                 return fail(SVMI_ERR_STATE, "batch plan: sweep does not match the circuit size");
             if (b.kind == BSTEP_ADJ && b.partial + sw.n_tiles * sw.n_grad > bp.n_grad_partials)
                 return fail(SVMI_ERR_STATE, "batch plan: bad gradient partial range");
+            if (sw.op_begin > sw.op_end || sw.op_end > bp.ops.size() || sw.n_xf > uint32_t(kMaxPhaseFactors))
+                return fail(SVMI_ERR_STATE, "batch plan: bad sweep op range");
+            for (uint32_t o = sw.op_begin; o < sw.op_end; o++)
+                if (bp.ops[o].h.type == DOP_PHASESET && bp.ops[o].h.s1 && bp.ops[o].h.grad >= sw.n_xf)
+                    return fail(SVMI_ERR_STATE, "batch plan: bad phase factor slot");
             for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
                 const PhaseInfo& ph = bp.phases[p];
-                if (ph.op_end > bp.ops.size() || ph.op_begin > ph.op_end)
+                if (ph.op_end > bp.ops.size() || ph.op_begin > ph.op_end || ph.op_begin < sw.op_begin ||
+                    ph.op_end > sw.op_end)
                     return fail(SVMI_ERR_STATE, "batch plan: bad phase");
                 for (int r = 0; r < kRegBits; r++)
                     if (ph.r[r] >= sw.k) return fail(SVMI_ERR_STATE, "batch plan: bad register bit");
@@ -376,6 +383,8 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
         for (SweepInfo sw : pr.sweeps) {
             sw.phase_begin += ph_base;
             sw.phase_end += ph_base;
+            sw.op_begin += op_base;
+            sw.op_end += op_base;
             sw.hitab += ht_base;
             bp.sweeps.push_back(sw);
         }
@@ -487,6 +496,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
             bp.steps.push_back(b);
             const uint32_t k = tile_k(b);
             const uint32_t ng = b.kind == BSTEP_ADJ ? bp.sweeps[b.sweep].n_grad : 0u;
+            const uint32_t nx = (b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ) ? bp.sweeps[b.sweep].n_xf : 0u;
             if (!bp.launches.empty() && bp.launches.back().step_end == idx &&
                 bp.launches.back().step_begin >= bp.level_begin.back() && bp.launches.back().kind == b.kind &&
                 bp.launches.back().k == k && bp.launches.back().init == b.init_zero) {
@@ -494,6 +504,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
                 L.step_end = idx + 1;
                 L.n_tasks += b.n_tasks;
                 L.max_grad = std::max(L.max_grad, ng);
+                L.max_xf = std::max(L.max_xf, nx);
             } else {
                 BatchLaunch L;
                 std::memset(&L, 0, sizeof(L));
@@ -502,6 +513,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
                 L.kind = b.kind;
                 L.k = k;
                 L.max_grad = ng;
+                L.max_xf = nx;
                 L.init = b.init_zero;
                 L.task_begin = b.task_begin;
                 L.n_tasks = b.n_tasks;
@@ -544,7 +556,7 @@ struct svmi_sim {
     DeviceProgram dprog;
     PlanOptions opt;
     bool profile = false;
-    int pipeline = 1;
+    int pipeline = 0;
     std::vector<hipEvent_t> prof_events;
     svmi_stats stats{};
     // backward program of the uploaded circuit for the last parameter map
@@ -1193,6 +1205,9 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
     std::vector<uint64_t> gidx(tsize);
     for (uint32_t i = 0; i < sw.n_grad; i++) gsum_tile[i] = 0.0;
     const uint64_t base = tile_base(sw, tile);
+    std::vector<c64> xf(std::max<uint32_t>(sw.n_xf, 1));
+    for (uint32_t o = sw.op_begin; o < sw.op_end; o++)
+        if (ops[o].h.type == DOP_PHASESET && ops[o].h.s1) xf[ops[o].h.grad] = phase_factor(ops[o].h, coef, base);
     for (uint32_t l = 0; l < tsize; l++) {
         gidx[l] = base | hitab[sw.hitab + (l >> sw.c)] | (l & cmask);
         t0[l] = init_zero ? c64{gidx[l] == 0 ? 1.0 : 0.0, 0.0} : s0[gidx[l]];
@@ -1214,8 +1229,8 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
                 if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG) {
                     gsum_tile[op.h.grad] += exec_grad_op(op.h, &op, a, b, coef, goff, off, base);
                 } else {
-                    exec_state_op(op.h, &op, a, coef, goff, off, base);
-                    if (ns == 2) exec_state_op(op.h, &op, b, coef, goff, off, base);
+                    exec_state_op(op.h, &op, a, coef, goff, off, base, xf.data());
+                    if (ns == 2) exec_state_op(op.h, &op, b, coef, goff, off, base, xf.data());
                 }
             }
             for (int r = 0; r < kRegs; r++) {

@@ -158,10 +158,15 @@ SVMI_HD inline void apply_diag(const DOp& op, c64 (&a)[kRegs], const c64* __rest
 // off[rho] = tile-local index bits of register rho, base = tile base index.
 // Product of controlled phases (h.s0 entries) applied to registers h.rsel.
 SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
-                                   uint64_t base)
+                                   uint64_t base, const c64* xf)
 {
     const PhaseEntry* pe = reinterpret_cast<const PhaseEntry*>(coef + h.coef);
     double pr = 1.0, pi = 0.0;
+    if (h.s1) {   // the tile's product of the entries without lane condition
+        const c64 f = xf[h.grad];
+        pr = f.re;
+        pi = f.im;
+    }
     const uint32_t cnt = h.s0;
 #pragma unroll 4
     for (uint32_t e = 0; e < cnt; e++) {
@@ -173,20 +178,6 @@ SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[kRegs], const c64* 
         pr = on ? tr : pr;
         pi = on ? ti : pi;
     }
-    // tables over bits outside the tile: index and factor are tile-uniform
-    const c64* tab = coef + h.coef + 2 * cnt;
-    for (uint32_t c = 0; c < h.s1; c++) {
-        const PhaseChunk* hc = reinterpret_cast<const PhaseChunk*>(tab);
-        const uint32_t nb = hc->nbits;
-        uint32_t idx = 0;
-        for (uint32_t i = 0; i < nb; i++) idx |= uint32_t((base >> hc->pos[i]) & 1u) << i;
-        const c64 f = tab[1 + idx];
-        const double tr = SVMI_FMA(pr, f.re, -(pi * f.im));
-        const double ti = SVMI_FMA(pr, f.im, pi * f.re);
-        pr = tr;
-        pi = ti;
-        tab += 1 + (1u << nb);
-    }
     const c64 ph{pr, pi};
     const uint32_t rsel = h.rsel;
 #pragma unroll
@@ -194,10 +185,27 @@ SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[kRegs], const c64* 
         if ((rsel >> r) & 1u) a[r] = cmul(a[r], ph);
 }
 
+// Factor slot of a DOP_PHASESET op for the tile at `base`: the product of
+// its h.s1 entries that follow the per-lane ones.
+SVMI_HD inline c64 phase_factor(const DOpHdr& h, const c64* __restrict__ coef, uint64_t base)
+{
+    const PhaseEntry* pe = reinterpret_cast<const PhaseEntry*>(coef + h.coef) + h.s0;
+    double pr = 1.0, pi = 0.0;
+    for (uint32_t e = 0; e < h.s1; e++) {
+        const PhaseEntry E = pe[e];
+        if ((base & E.emask) != E.emask) continue;
+        const double tr = SVMI_FMA(pr, E.ph_re, -(pi * E.ph_im));
+        const double ti = SVMI_FMA(pr, E.ph_im, pi * E.ph_re);
+        pr = tr;
+        pi = ti;
+    }
+    return c64{pr, pi};
+}
+
 // h = the op's header (prefetched), op = the op in memory (diagonal tail).
 SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[kRegs],
                                   const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
-                                  uint64_t base)
+                                  uint64_t base, const c64* xf)
 {
     switch (h.type) {
         SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, dispatch_dense(h.smask, a, coef + h.coef))
@@ -207,7 +215,7 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_CASE_ID(DOP_SRN, DOP_SRN + 100, dispatch_srn(h.s0, a))
         SVMI_CASE_ID(DOP_CPHASE, DOP_CPHASE + 100, apply_cphase(h, a, coef, goff, base))
         SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag(*op, a, coef, goff, off, base))
-        SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset(h, a, coef, goff, base))
+        SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset(h, a, coef, goff, base, xf))
     default: break;
     }
 }

@@ -88,16 +88,10 @@ struct PhaseEntry {
 };
 static_assert(sizeof(PhaseEntry) == 32, "PhaseEntry layout");
 
-// Phase-product table header (one coefficient slot), followed by 2^nbits
-// products: entry idx = product of the phases of the bits set in idx, bit i
-// of idx <-> global index bit pos[i].
-constexpr int kPhaseChunkBits = 8;
-struct PhaseChunk {
-    uint8_t pos[kPhaseChunkBits];
-    uint32_t nbits;
-    uint32_t pad;
-};
-static_assert(sizeof(PhaseChunk) == 16, "PhaseChunk layout");
+// Phase products (DOP_PHASESET): h.s0 per-lane entries, then h.s1 entries
+// with no condition inside the tile (gmask == 0), whose product depends on
+// the tile only; it is computed once per tile into factor slot h.grad.
+constexpr int kMaxPhaseFactors = 64;   // factor slots per sweep (LDS: 16 B each)
 
 // Which state(s) an op applies to in adjoint sweeps.
 enum : uint8_t { DOP_TO_PSI = 1, DOP_TO_LAM = 2, DOP_TO_BOTH = 3 };
@@ -139,6 +133,10 @@ struct SweepInfo {
     uint32_t grad_base;  // adjoint: first global accumulator of this sweep
     uint64_t extmask;    // physical qubits outside the tile
     uint64_t n_tiles;
+    uint32_t n_xf;       // tile-uniform phase factors (LDS slots) of this sweep
+    uint32_t op_begin;   // the sweep's ops are [op_begin, op_end)
+    uint32_t op_end;
+    uint32_t pad;
 };
 
 // A standalone op (run by its own kernel on the whole state, outside sweeps).

@@ -66,8 +66,9 @@ tile_store(const c64* t, c64* __restrict__ s, const SweepInfo& sw, const uint64_
 // (and its register groups out of scratch).  Ends with a barrier.
 template <int NS, bool kIdle>
 __device__ __attribute__((always_inline)) inline void
-tile_phases(c64* t0, c64* t1, double* gacc, const SweepInfo& sw, const PhaseInfo* __restrict__ phases,
-            const DOp* __restrict__ ops, const c64* __restrict__ coef, uint64_t base, uint32_t tid, uint32_t nt)
+tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
+            const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops, const c64* __restrict__ coef,
+            uint64_t base, uint32_t tid, uint32_t nt)
 {
     const bool active = !kIdle || tid < nt;
     const uint32_t nwaves = (nt + 63) / 64;
@@ -97,8 +98,8 @@ tile_phases(c64* t0, c64* t1, double* gacc, const SweepInfo& sw, const PhaseInfo
                     for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
                     if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
                 } else {
-                    exec_state_op(h, ops + o, a, coef, goff, off, base);
-                    if (NS == 2) exec_state_op(h, ops + o, b, coef, goff, off, base);
+                    exec_state_op(h, ops + o, a, coef, goff, off, base, xf);
+                    if (NS == 2) exec_state_op(h, ops + o, b, coef, goff, off, base, xf);
                 }
             }
 #pragma unroll
@@ -111,6 +112,20 @@ tile_phases(c64* t0, c64* t1, double* gacc, const SweepInfo& sw, const PhaseInfo
     }
 }
 
+// The tile's phase factors (one per factor slot of the sweep) into LDS; the
+// caller's next barrier publishes them.
+__device__ __attribute__((always_inline)) inline void
+tile_factors(c64* xf, const SweepInfo& sw, const DOp* __restrict__ ops, const c64* __restrict__ coef,
+             uint64_t base, uint32_t tid)
+{
+    if (sw.n_xf == 0) return;
+    for (uint32_t o = sw.op_begin + tid; o < sw.op_end; o += blockDim.x) {
+        const DOpHdr h = ops[o].h;
+        if (h.type == DOP_PHASESET && h.s1) xf[h.grad] = phase_factor(h, coef, base);
+    }
+}
+
+// LDS of a tile: t0 [t1] | phase factors | gradient accumulators.
 // One tile of one sweep: load, phases, store (and for NS = 2 the tile's
 // gradient partials).  init_zero: the tile is generated as |0..0>.
 template <int NS, bool kIdle>
@@ -124,11 +139,13 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     const uint32_t tsize = 1u << sw.k;
     c64* t0 = reinterpret_cast<c64*>(smem);
     c64* t1 = t0 + (NS == 2 ? tsize : 0);
-    double* gacc = reinterpret_cast<double*>(t0 + tsize * NS);
+    c64* xf = t0 + tsize * NS;
+    double* gacc = reinterpret_cast<double*>(xf + sw.n_xf);
     const uint32_t tid = threadIdx.x;
     const bool active = !kIdle || tid < nt;
     const uint32_t nwaves = (nt + 63) / 64;
     const uint64_t* ht = hitab + sw.hitab;
+    const uint64_t base = tile_base(sw, tile);
 
     if (active) {
         c64 r[kRegs];
@@ -141,8 +158,9 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     }
     if (NS == 2)
         for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
+    tile_factors(xf, sw, ops, coef, base, tid);
     __syncthreads();
-    tile_phases<NS, kIdle>(t0, t1, gacc, sw, phases, ops, coef, tile_base(sw, tile), tid, nt);
+    tile_phases<NS, kIdle>(t0, t1, gacc, xf, sw, phases, ops, coef, base, tid, nt);
     if (active) {
         tile_store(t0, s0, sw, ht, tile, tid, nt);
         if (NS == 2) tile_store(t1, s1, sw, ht, tile, tid, nt);
@@ -185,18 +203,21 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
 {
     extern __shared__ __align__(16) unsigned char smem[];
     c64* t0 = reinterpret_cast<c64*>(smem);
+    c64* xf = t0 + (size_t(1) << sw.k);
     const uint32_t tid = threadIdx.x, nt = blockDim.x;
     const uint64_t* ht = hitab + sw.hitab;
     uint64_t t = blockIdx.x;
     c64 nx[kRegs];
     tile_fetch(nx, s0, sw, ht, tile_offset + t, tid, nt, false);
     for (;;) {
+        const uint64_t base = tile_base(sw, tile_offset + t);
         tile_put(nx, t0, tid, nt);
+        tile_factors(xf, sw, ops, coef, base, tid);
         __syncthreads();
         const uint64_t tn = t + gridDim.x;
         const bool more = tn < n_tiles;
         if (more) tile_fetch(nx, s0, sw, ht, tile_offset + tn, tid, nt, false);
-        tile_phases<1, false>(t0, nullptr, nullptr, sw, phases, ops, coef, tile_base(sw, tile_offset + t), tid, nt);
+        tile_phases<1, false>(t0, nullptr, nullptr, xf, sw, phases, ops, coef, base, tid, nt);
         tile_store(t0, s0, sw, ht, tile_offset + t, tid, nt);
         if (!more) break;
         __syncthreads();  // the store's LDS reads finish before the next put
@@ -313,19 +334,19 @@ k_batch_pauli(BatchArgs a, uint32_t s0, uint32_t s1, uint64_t task0)
     if (threadIdx.x == 0) a.e_partials[st.partial + task] = red[0];
 }
 
-size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad)
+size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad, uint32_t max_xf)
 {
-    if (kind == BSTEP_FWD) return size_t(16) << k;
+    if (kind == BSTEP_FWD) return (size_t(16) << k) + size_t(16) * max_xf;
     if (kind == BSTEP_ADJ) {
         const size_t nwaves = ((size_t(1) << (k - kRegBits)) + 63) / 64;
-        return (size_t(32) << k) + size_t(8) * max_grad * nwaves;
+        return (size_t(32) << k) + size_t(16) * max_xf + size_t(8) * max_grad * nwaves;
     }
     return 0;
 }
 
 hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L)
 {
-    const size_t smem = batch_launch_smem(L.kind, L.k, L.max_grad);
+    const size_t smem = batch_launch_smem(L.kind, L.k, L.max_grad, L.max_xf);
     if (smem > g_max_smem) return hipErrorInvalidConfiguration;
     const uint32_t threads = (L.kind == BSTEP_FWD || L.kind == BSTEP_ADJ) ? 1u << (L.k - kRegBits) : kThreads;
     const uint64_t max_grid = 1ull << 30;
@@ -359,7 +380,7 @@ size_t sweep_smem_bytes(const SweepInfo& sw, int ns)
 {
     const size_t nt = size_t(1) << (sw.k - kRegBits);
     const size_t nwaves = (nt + 63) / 64;
-    return (size_t(16) << sw.k) * ns + size_t(8) * sw.n_grad * nwaves;
+    return (size_t(16) << sw.k) * ns + size_t(16) * sw.n_xf + size_t(8) * sw.n_grad * nwaves;
 }
 
 // Workgroups of k_sweep1p resident at once for tile size k (occupancy x CUs).
@@ -843,8 +864,11 @@ static size_t g_staged_smem = 0;   // dynamic LDS the staged sampling kernels ma
 
 hipError_t init_sample_kernels()
 {
-    const size_t want = sweep_smem_limit();
+    // the largest staged block: 2^14 amplitudes over 256 threads (the kernels
+    // also use a little static LDS, so ask for exactly this)
+    const size_t want = staged_smem(256, 64);
     g_staged_smem = 0;
+    if (want > sweep_smem_limit()) return hipSuccess;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_totals_staged),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) == hipSuccess &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sample_staged),

@@ -127,6 +127,7 @@ bool is_cphase(const SOp& s)
 
 struct Encoder {
     Program& prog;
+    uint32_t n_xf = 0;   // factor slots used in the current sweep
     explicit Encoder(Program& p) : prog(p) {}
 
     // ops [i, j) are controlled phases; emit one DOP_PHASESET per distinct
@@ -158,64 +159,30 @@ struct Encoder {
             groups[g].second.push_back(e);
         }
         for (auto& gr : groups) {
-            // entries conditioned on one bit outside the tile are uniform per
-            // tile: they go into product tables over chunks of up to 8 such
-            // bits (one table read per chunk on the device); the others stay
-            // per-entry
-            std::vector<PhaseEntry> rest;
-            std::vector<int> ebits;
-            std::vector<c64> ebit_phase;
-            for (const PhaseEntry& e : gr.second) {
-                if (e.gmask == 0 && __builtin_popcountll(e.emask) == 1) {
-                    const int b = __builtin_ctzll(e.emask);
-                    size_t i = 0;
-                    while (i < ebits.size() && ebits[i] != b) i++;
-                    const c64 ph{e.ph_re, e.ph_im};
-                    if (i == ebits.size()) {
-                        ebits.push_back(b);
-                        ebit_phase.push_back(ph);
-                    } else {
-                        ebit_phase[i] = cmul(ebit_phase[i], ph);
-                    }
-                } else {
-                    rest.push_back(e);
-                }
-            }
-            if (ebits.size() < 2) {
-                rest = gr.second;
-                ebits.clear();
+            // entries with no condition inside the tile depend on the tile
+            // only: their product is formed once per tile (factor slot)
+            std::vector<PhaseEntry> lane, uni;
+            for (const PhaseEntry& e : gr.second) (e.gmask == 0 ? uni : lane).push_back(e);
+            if (uni.empty() || n_xf >= uint32_t(kMaxPhaseFactors)) {
+                lane = gr.second;
+                uni.clear();
             }
             DOp d;
             std::memset(&d, 0, sizeof(d));
             d.targets = prog.n_states == 2 ? DOP_TO_BOTH : DOP_TO_PSI;
             d.h.type = DOP_PHASESET;
             d.h.rsel = gr.first;
-            d.h.s0 = uint32_t(rest.size());
+            d.h.s0 = uint32_t(lane.size());
+            d.h.s1 = uint32_t(uni.size());
+            d.h.grad = uni.empty() ? 0u : n_xf++;
             d.h.coef = uint32_t(prog.coef.size());
-            for (const PhaseEntry& e : rest) {
-                c64 two[2];
-                std::memcpy(two, &e, sizeof(e));
-                prog.coef.push_back(two[0]);
-                prog.coef.push_back(two[1]);
-            }
-            uint32_t chunks = 0;
-            for (size_t c0 = 0; c0 < ebits.size(); c0 += kPhaseChunkBits, chunks++) {
-                const uint32_t nb = uint32_t(std::min<size_t>(kPhaseChunkBits, ebits.size() - c0));
-                PhaseChunk hdr;
-                std::memset(&hdr, 0, sizeof(hdr));
-                for (uint32_t i = 0; i < nb; i++) hdr.pos[i] = uint8_t(ebits[c0 + i]);
-                hdr.nbits = nb;
-                c64 slot;
-                std::memcpy(&slot, &hdr, sizeof(slot));
-                prog.coef.push_back(slot);
-                for (uint32_t idx = 0; idx < (1u << nb); idx++) {
-                    c64 f{1.0, 0.0};
-                    for (uint32_t i = 0; i < nb; i++)
-                        if ((idx >> i) & 1) f = cmul(f, ebit_phase[c0 + i]);
-                    prog.coef.push_back(f);
+            for (const std::vector<PhaseEntry>* v : {&lane, &uni})
+                for (const PhaseEntry& e : *v) {
+                    c64 two[2];
+                    std::memcpy(two, &e, sizeof(e));
+                    prog.coef.push_back(two[0]);
+                    prog.coef.push_back(two[1]);
                 }
-            }
-            d.h.s1 = chunks;
             prog.ops.push_back(d);
         }
     }
@@ -509,6 +476,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     sw.k = uint32_t(k);
     sw.c = uint32_t(c);
     sw.phase_begin = uint32_t(prog.phases.size());
+    sw.op_begin = uint32_t(prog.ops.size());
     sw.hitab = uint32_t(prog.hitab.size());
     sw.n_ext = 0;
     sw.extmask = 0;
@@ -562,7 +530,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             }
             if ((lm & ~R) == 0) {
                 if (ph) ph->push_back(s);
-                score += 8 * reg_cost(*s->op);                           // progress first
+                score += 8 * (4L << s->op->k);                           // progress first (any op
+                                                                         // taken saves a phase later)
             } else {
                 blocked |= s->dep;
                 if (rest) rest->push_back(s);
@@ -644,6 +613,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     }
     sw.phase_end = uint32_t(prog.phases.size());
     sw.n_grad = n_grad_sweep;
+    sw.n_xf = enc.n_xf;
+    sw.op_end = uint32_t(prog.ops.size());
     prog.n_grad += n_grad_sweep;
     prog.steps.push_back(Step{Step::SWEEP, uint32_t(prog.sweeps.size())});
     prog.sweeps.push_back(sw);
@@ -799,7 +770,7 @@ std::string describe_program(const Program& prog)
                     ss << " " << nm[d.h.type];
                     if (d.h.type == DOP_PHASESET) {
                         ss << d.h.s0;
-                        if (d.h.s1) ss << "t" << d.h.s1;
+                        if (d.h.s1) ss << "u" << d.h.s1;
                     }
                     if (d.h.type == DOP_DENSE || d.h.type == DOP_GRAD_DENSE) ss << __builtin_popcount(d.h.smask);
                 }

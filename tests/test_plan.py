@@ -144,12 +144,12 @@ def test_batch_rejects_five_qubit_gates():
 
 
 @pytest.mark.parametrize("n,tile,low", [(16, 6, 2), (18, 8, 4), (17, 12, 4)])
-def test_phase_product_tables_match_oracle(n, tile, low):
+def test_phase_factors_match_oracle(n, tile, low):
     # QFT-pattern circuits put many controlled phases on qubits outside the
-    # tile; those become per-tile product tables (PS..t..) -- check the plan
-    # really uses them and that the result is unchanged
+    # tile; their product is formed once per tile (PS..u.. ops) -- check the
+    # plan really uses that and that the result is unchanged
     gates = W.qft(n=n)
     _, desc = svmi.plan(n, gates, tile_qubits=tile, low_qubits=low)
-    assert "t" in "".join(tok for tok in desc.split() if tok.startswith("PS"))
+    assert "u" in "".join(tok for tok in desc.split() if tok.startswith("PS"))
     st = W.random_state(n, seed=n + 40)
     assert _err(svmi.emulate(n, gates, st, tile_qubits=tile, low_qubits=low), O.run(n, gates, st)) < 1e-12
