@@ -1119,7 +1119,7 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
         break;
     case SVMI_OPT_PROFILE: h->profile = value != 0; return SVMI_OK;
     case SVMI_OPT_PIPELINE:
-        if (value < 0 || value > 2) return fail(SVMI_ERR_ARG, "pipeline must be 0, 1 or 2");
+        if (value < 0 || value > 64) return fail(SVMI_ERR_ARG, "pipeline must be in [0, 64]");
         h->pipeline = int(value);
         return SVMI_OK;
     default: return fail(SVMI_ERR_ARG, "unknown option " + std::to_string(option));

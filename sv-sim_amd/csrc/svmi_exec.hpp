@@ -23,10 +23,15 @@ namespace svmi {
         break;
 #define SVMI_CASE(N, STMT) SVMI_CASE_ID(N, N, STMT)
 
+constexpr int cpopc(int v) { return v ? (v & 1) + cpopc(v >> 1) : 0; }
+
+// kMaxQ: widest dense op compiled in (the planner emits no wider one for the
+// kernel instantiated with it)
+template <int kMaxQ>
 SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[kRegs], const c64* __restrict__ M)
 {
     switch (smask) {
-#define SVMI_DENSE_CASE(S) SVMI_CASE(S, apply_dense<S>(a, M))
+#define SVMI_DENSE_CASE(S) SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ) apply_dense<S>(a, M))
         SVMI_DENSE_CASE(1) SVMI_DENSE_CASE(2) SVMI_DENSE_CASE(3) SVMI_DENSE_CASE(4)
         SVMI_DENSE_CASE(5) SVMI_DENSE_CASE(6) SVMI_DENSE_CASE(7) SVMI_DENSE_CASE(8)
         SVMI_DENSE_CASE(9) SVMI_DENSE_CASE(10) SVMI_DENSE_CASE(11) SVMI_DENSE_CASE(12)
@@ -36,12 +41,13 @@ SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[kRegs], const c64* __rest
     }
 }
 
+template <int kMaxQ>
 SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&p)[kRegs], const c64 (&l)[kRegs],
                                           const c64* __restrict__ M)
 {
     double sum = 0.0;
     switch (smask) {
-#define SVMI_GRAD_CASE(S) SVMI_CASE(S, sum = grad_dense<S>(p, l, M))
+#define SVMI_GRAD_CASE(S) SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ) sum = grad_dense<S>(p, l, M))
         SVMI_GRAD_CASE(1) SVMI_GRAD_CASE(2) SVMI_GRAD_CASE(3) SVMI_GRAD_CASE(4)
         SVMI_GRAD_CASE(5) SVMI_GRAD_CASE(6) SVMI_GRAD_CASE(7) SVMI_GRAD_CASE(8)
         SVMI_GRAD_CASE(9) SVMI_GRAD_CASE(10) SVMI_GRAD_CASE(11) SVMI_GRAD_CASE(12)
@@ -203,12 +209,13 @@ SVMI_HD inline c64 phase_factor(const DOpHdr& h, const c64* __restrict__ coef, u
 }
 
 // h = the op's header (prefetched), op = the op in memory (diagonal tail).
+template <int kMaxQ = kRegBits>
 SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[kRegs],
                                   const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
                                   uint64_t base, const c64* xf)
 {
     switch (h.type) {
-        SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, dispatch_dense(h.smask, a, coef + h.coef))
+        SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, dispatch_dense<kMaxQ>(h.smask, a, coef + h.coef))
         SVMI_CASE_ID(DOP_X, DOP_X + 100, dispatch_x(h.s0, a))
         SVMI_CASE_ID(DOP_CX, DOP_CX + 100, dispatch_cx(h.s0, h.s1, a))
         SVMI_CASE_ID(DOP_SWAP, DOP_SWAP + 100, dispatch_swap(h.s0, h.s1, a))
@@ -221,11 +228,12 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
 }
 
 // Gradient op: Re sum conj(lam) K phi over this group.
+template <int kMaxQ = kRegBits>
 SVMI_HD inline double exec_grad_op(const DOpHdr& h, const DOp* __restrict__ op, const c64 (&p)[kRegs],
                                    const c64 (&l)[kRegs], const c64* __restrict__ coef, uint32_t goff,
                                    const uint32_t (&off)[kRegs], uint64_t base)
 {
-    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense(h.smask, p, l, coef + h.coef);
+    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense<kMaxQ>(h.smask, p, l, coef + h.coef);
     // DOP_GRAD_DIAG
     const uint32_t e = diag_ext_index(*op, base);
     const c64* tab = coef + h.coef;

@@ -57,6 +57,8 @@ constexpr int kRegBits = 4;                 // register bits per phase
 constexpr int kRegs = 1 << kRegBits;        // amplitudes per thread per group
 constexpr int kMaxTileQubits = 13;          // LDS tile: 2^13 * 16 B = 128 KiB
 constexpr int kMaxTileQubitsAdj = 11;       // two states (phi, lambda): 2 * 32 KiB
+constexpr int kMaxDenseQubitsAdj = 2;       // two-state sweeps run dense ops of <= 2 qubits
+                                            // (wider gates: full-state passes on both states)
 constexpr int kThreads = 256;               // threads of the reduction / Pauli workgroups
 constexpr int kMaxTileThreads = 1 << (kMaxTileQubits - kRegBits);   // 512
 constexpr int kMaxDenseQubits = 5;          // standalone dense ops

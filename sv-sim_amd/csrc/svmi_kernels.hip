@@ -93,13 +93,13 @@ tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
                 const DOpHdr h = nxt;
                 if (o + 1 < ph.op_end) nxt = ops[o + 1].h;
                 if (NS == 2 && (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG)) {
-                    double v = exec_grad_op(h, ops + o, a, b, coef, goff, off, base);
+                    double v = exec_grad_op<kMaxDenseQubitsAdj>(h, ops + o, a, b, coef, goff, off, base);
                     const int width = nt < 64 ? int(nt) : 64;
                     for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
                     if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
                 } else {
-                    exec_state_op(h, ops + o, a, coef, goff, off, base, xf);
-                    if (NS == 2) exec_state_op(h, ops + o, b, coef, goff, off, base, xf);
+                    exec_state_op<NS == 2 ? kMaxDenseQubitsAdj : kRegBits>(h, ops + o, a, coef, goff, off, base, xf);
+                    if (NS == 2) exec_state_op<kMaxDenseQubitsAdj>(h, ops + o, b, coef, goff, off, base, xf);
                 }
             }
 #pragma unroll
@@ -178,12 +178,17 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
 
 // One state, one tile per workgroup: up to 8 waves per CU (one 2^13 tile of
 // 512 threads, or two 2^12 tiles of 256), so 256 VGPRs per thread.
+// stagger_lo..stagger_hi: workgroups of the first resident wave that start
+// later (by `stagger` sleeps of 127*64 cycles), so that the co-resident
+// workgroups of a CU alternate loading and computing.
 __global__ void __launch_bounds__(kMaxTileThreads, 1)
 k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
-         uint64_t tile_offset)
+         uint64_t tile_offset, uint32_t stagger_lo, uint32_t stagger_hi, uint32_t stagger)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    if (blockIdx.x >= stagger_lo && blockIdx.x < stagger_hi)
+        for (uint32_t i = 0; i < stagger; i++) __builtin_amdgcn_s_sleep(127);
     sweep_tile<1, false>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x,
                          smem, false);
 }
@@ -225,7 +230,8 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
     }
 }
 
-// Two states (adjoint): one wave per SIMD, registers for both groups.
+// Two states (adjoint): two register groups per thread (one wave per SIMD),
+// dense ops of at most 2 qubits.
 __global__ void __launch_bounds__(kThreads, 1)
 k_sweep2(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
@@ -409,7 +415,7 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
     const uint32_t nt = 1u << (sw.k - kRegBits);
     const size_t smem = sweep_smem_bytes(sw, ns);
     if (smem > g_max_smem) return hipErrorInvalidConfiguration;
-    if (ns == 1 && pipeline) {
+    if (ns == 1 && (pipeline == 1 || pipeline == 2)) {
         const bool wide = pipeline == 2 && nt <= 256;
         const uint64_t resident = wide ? pipeline_grid<true>(sw.k) : pipeline_grid<false>(sw.k);
         if (sw.n_tiles > resident) {
@@ -425,9 +431,18 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
     const uint64_t max_grid = 1ull << 30;
     for (uint64_t t0 = 0; t0 < sw.n_tiles; t0 += max_grid) {
         const uint64_t cnt = sw.n_tiles - t0 < max_grid ? sw.n_tiles - t0 : max_grid;
-        if (ns == 1)
+        if (ns == 1) {
+            // pipeline 3: stagger the second workgroup slot of every CU
+            uint32_t lo = 0, hi = 0, sl = 0;
+            if (pipeline >= 3) {
+                const uint64_t res = pipeline_grid<false>(sw.k);
+                lo = uint32_t(res / 2);
+                hi = uint32_t(res);
+                sl = uint32_t(pipeline - 2);
+            }
             hipLaunchKernelGGL(k_sweep1, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, sw, phases, ops, coef,
-                               hitab, t0);
+                               hitab, t0, lo, hi, sl);
+        }
         else
             hipLaunchKernelGGL(k_sweep2, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases, ops,
                                coef, hitab, grad_partials, t0);

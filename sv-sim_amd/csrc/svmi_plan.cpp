@@ -69,7 +69,8 @@ bool is_identity(const LOp& op)
     return true;
 }
 
-SOp make_sop(const LOp* op, bool grad, int param)
+// max_q: widest dense op a sweep may run in registers (wider: full-state pass)
+SOp make_sop(const LOp* op, bool grad, int param, int max_q = kRegBits)
 {
     SOp s;
     s.op = op;
@@ -80,7 +81,7 @@ SOp make_sop(const LOp* op, bool grad, int param)
     // does not commute with complex phases even on other qubits.
     s.dep = op->kind == LOp::SRN ? ~uint64_t(0) : s.mask;
     s.diag = op->kind == LOp::DIAG;
-    s.standalone = op->kind == LOp::DENSE && op->k > kRegBits;
+    s.standalone = op->kind == LOp::DENSE && op->k > max_q && !grad;
     s.identity = !grad && is_identity(*op);
     return s;
 }
@@ -357,6 +358,7 @@ struct RunMerger {
     std::deque<LOp>& owned;
     std::vector<const SOp*>& out;
     std::deque<SOp>& sowned;
+    int max_q;                 // widest merged op
     std::vector<const SOp*> run;
     uint64_t run_mask = 0;
 
@@ -385,7 +387,7 @@ struct RunMerger {
             separate += reg_cost(*s->op) + kOpOverhead;
         }
         const int q = __builtin_popcount(S);
-        if ((4L << q) + kOpOverhead > separate) {
+        if (q > max_q || (4L << q) + kOpOverhead > separate) {
             for (const SOp* s : run) out.push_back(s);
         } else {
             const size_t D = size_t(1) << q;
@@ -584,7 +586,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         for (int sl = 0; sl < kRegBits; sl++) phys_of_slot[sl] = tpos[pi.r[sl]];
         std::vector<const SOp*> merged;
         {
-            RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged, sowned, {}, 0};
+            RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged, sowned,
+                         prog.n_states == 2 ? kMaxDenseQubitsAdj : kRegBits, {}, 0};
             for (const SOp* s : ph) rm.add(s);
             rm.flush();
         }
@@ -732,7 +735,7 @@ void build_adjoint_program(const std::vector<AdjEntry>& entries, int n, const Pl
     prog.n_states = 2;
     std::vector<SOp> sops;
     sops.reserve(entries.size());
-    for (const AdjEntry& e : entries) sops.push_back(make_sop(&e.op, e.grad, e.param));
+    for (const AdjEntry& e : entries) sops.push_back(make_sop(&e.op, e.grad, e.param, kMaxDenseQubitsAdj));
     PlanOptions o = opt;
     o.fusion = true;  // adjoint always runs as tiled sweeps
     schedule(sops, n, o, prog);
