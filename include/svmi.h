@@ -95,8 +95,11 @@ typedef enum svmi_option {
     SVMI_OPT_LOW_QUBITS = 2,  /* low qubits every tile keeps for coalescing (2..6, default 4) */
     SVMI_OPT_SWEEP_FLOPS = 3, /* per-amplitude FMA budget of one sweep (0 = unlimited)      */
     SVMI_OPT_PROFILE = 4,     /* 1 = time every launch with HIP events (svmi_stats)         */
-    SVMI_OPT_PIPELINE = 5     /* 1 = persistent sweeps prefetching the next tile,
+    SVMI_OPT_PIPELINE = 5,    /* 1 = persistent sweeps prefetching the next tile,
                                  2 = the same with one 512-VGPR workgroup per CU, 0 = off (default) */
+    SVMI_OPT_OP_COST = 6,     /* fusion cost model: per-op cost in FMA per amplitude (default 8) */
+    SVMI_OPT_COEF_BUDGET = 7  /* split a forward sweep whose op coefficients exceed this many
+                                 bytes into consecutive sweeps (default 20000, 0 = never)   */
 } svmi_option;
 
 /* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,

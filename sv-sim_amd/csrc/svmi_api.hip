@@ -1117,6 +1117,14 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
         if (value < 0) return fail(SVMI_ERR_ARG, "sweep budget must be >= 0");
         h->opt.sweep_budget = long(value);
         break;
+    case SVMI_OPT_OP_COST:
+        if (value < 0 || value > 1024) return fail(SVMI_ERR_ARG, "op cost must be in [0, 1024]");
+        h->opt.op_overhead = long(value);
+        break;
+    case SVMI_OPT_COEF_BUDGET:
+        if (value < 0) return fail(SVMI_ERR_ARG, "coefficient budget must be >= 0");
+        h->opt.coef_budget = long(value);
+        break;
     case SVMI_OPT_PROFILE: h->profile = value != 0; return SVMI_OK;
     case SVMI_OPT_PIPELINE:
         if (value < 0 || value > 64) return fail(SVMI_ERR_ARG, "pipeline must be in [0, 64]");

@@ -307,7 +307,6 @@ struct Encoder {
 // share no qubit with it and are emitted ahead of it; otherwise the run is
 // flushed first.  SRN (real-linear) always flushes.
 // ---------------------------------------------------------------------------
-constexpr long kOpOverhead = 8;  // per-op decode cost in FMA per amplitude
 
 bool is_register_perm(const LOp& op);
 
@@ -359,6 +358,7 @@ struct RunMerger {
     std::vector<const SOp*>& out;
     std::deque<SOp>& sowned;
     int max_q;                 // widest merged op
+    long overhead;             // per-op cost on top of its FMAs
     std::vector<const SOp*> run;
     uint64_t run_mask = 0;
 
@@ -384,10 +384,10 @@ struct RunMerger {
         long separate = 0;
         for (const SOp* s : run) {
             for (int i = 0; i < s->op->k; i++) S |= 1u << slot_of[tloc[s->op->q[i]]];
-            separate += reg_cost(*s->op) + kOpOverhead;
+            separate += reg_cost(*s->op) + overhead;
         }
         const int q = __builtin_popcount(S);
-        if (q > max_q || (4L << q) + kOpOverhead > separate) {
+        if (q > max_q || (4L << q) + overhead > separate) {
             for (const SOp* s : run) out.push_back(s);
         } else {
             const size_t D = size_t(1) << q;
@@ -458,7 +458,8 @@ void emit_standalone(Program& prog, const SOp& s)
     prog.standalone.push_back(so);
 }
 
-void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T, int n, int c)
+void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T, int n, int c,
+                const PlanOptions& opt)
 {
     Encoder enc(prog);
     std::deque<LOp> owned;     // merged ops (live until encoded)
@@ -587,7 +588,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         std::vector<const SOp*> merged;
         {
             RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged, sowned,
-                         prog.n_states == 2 ? kMaxDenseQubitsAdj : kRegBits, {}, 0};
+                         prog.n_states == 2 ? kMaxDenseQubitsAdj : kRegBits, opt.op_overhead, {}, 0};
             for (const SOp* s : ph) rm.add(s);
             rm.flush();
         }
@@ -619,6 +620,40 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     sw.n_xf = enc.n_xf;
     sw.op_end = uint32_t(prog.ops.size());
     prog.n_grad += n_grad_sweep;
+    // The coefficients a tile reads while it runs the sweep go through the
+    // small scalar cache that a few CUs share; when their total exceeds
+    // opt.coef_budget, run the phases as several consecutive sweeps over the
+    // same tile layout (one more HBM pass each, far fewer cache misses).
+    if (prog.n_states == 1 && opt.coef_budget > 0) {
+        auto phase_bytes = [&](const PhaseInfo& ph) {
+            long b = 0;
+            for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
+                const DOpHdr& h = prog.ops[o].h;
+                if (h.type == DOP_DENSE) b += 16L << (2 * __builtin_popcount(h.smask));
+                else if (h.type == DOP_PHASESET) b += 32L * (h.s0 + h.s1);
+                else b += 40;
+            }
+            return b;
+        };
+        uint32_t p0 = sw.phase_begin;
+        long acc = 0;
+        for (uint32_t p = sw.phase_begin; p <= sw.phase_end; p++) {
+            const long pb = p < sw.phase_end ? phase_bytes(prog.phases[p]) : 0;
+            if (p == sw.phase_end || (p > p0 && acc + pb > opt.coef_budget)) {
+                SweepInfo part = sw;
+                part.phase_begin = p0;
+                part.phase_end = p;
+                part.op_begin = prog.phases[p0].op_begin;
+                part.op_end = prog.phases[p - 1].op_end;
+                prog.steps.push_back(Step{Step::SWEEP, uint32_t(prog.sweeps.size())});
+                prog.sweeps.push_back(part);
+                p0 = p;
+                acc = 0;
+            }
+            acc += pb;
+        }
+        return;
+    }
     prog.steps.push_back(Step{Step::SWEEP, uint32_t(prog.sweeps.size())});
     prog.sweeps.push_back(sw);
 }
@@ -691,7 +726,7 @@ void schedule(std::vector<SOp>& sops, int n, const PlanOptions& opt, Program& pr
         }
         // pad T to kk qubits with the lowest free positions
         for (int p = 0; p < n && popc(T) < kk; p++) T |= uint64_t(1) << p;
-        emit_sweep(prog, taken, T, n, c);
+        emit_sweep(prog, taken, T, n, c, opt);
         pending.swap(rest);
     }
 }

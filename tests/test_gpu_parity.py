@@ -29,7 +29,8 @@ def _gpu_run(n, gates, state=None, **opts):
     with svmi.Engine(n) as e:
         for k, v in opts.items():
             e.set_option({"fusion": svmi.OPT_FUSION, "tile": svmi.OPT_TILE_QUBITS, "low": svmi.OPT_LOW_QUBITS,
-                          "budget": svmi.OPT_SWEEP_FLOPS}[k], v)
+                          "budget": svmi.OPT_SWEEP_FLOPS, "coef": svmi.OPT_COEF_BUDGET,
+                          "opcost": svmi.OPT_OP_COST, "pipeline": svmi.OPT_PIPELINE}[k], v)
         if state is not None:
             e.set_state(*state)
         e.upload(gates)
@@ -79,7 +80,8 @@ def test_gate_kats_all_ops():
 
 
 @pytest.mark.parametrize("n", [1, 3, 4, 6, 9, 13, 15])
-@pytest.mark.parametrize("opts", [{}, {"fusion": 0}, {"tile": 6, "low": 2}, {"tile": 8, "low": 0}, {"tile": 11}, {"tile": 12}])
+@pytest.mark.parametrize("opts", [{}, {"fusion": 0}, {"tile": 6, "low": 2}, {"tile": 8, "low": 0}, {"tile": 11}, {"tile": 12},
+                                  {"coef": 2000}, {"opcost": 0}, {"tile": 13}])
 def test_random_all_op_circuits(n, opts):
     rng = np.random.default_rng(500 + n)
     gates = []
@@ -95,7 +97,10 @@ def test_random_all_op_circuits(n, opts):
 @pytest.mark.parametrize("n", [14, 20])
 def test_u3_brickwall_and_qft_vs_oracle(n):
     g = W.u3_brickwall(n=n, layers=20)
-    assert _err(_gpu_run(n, g), O.run(n, g)) < TOL
+    want = O.run(n, g)
+    assert _err(_gpu_run(n, g), want) < TOL
+    assert _err(_gpu_run(n, g, coef=6000), want) < TOL        # sweeps split by coefficient bytes
+    assert _err(_gpu_run(n, g, pipeline=1), want) < TOL       # persistent prefetching sweeps
     st = W.random_state(n, seed=3000 + n)
     q = W.qft(n=n)
     assert _err(_gpu_run(n, q, st), O.run(n, q, st)) < TOL
