@@ -624,7 +624,9 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     // small scalar cache that a few CUs share; when their total exceeds
     // opt.coef_budget, run the phases as several consecutive sweeps over the
     // same tile layout (one more HBM pass each, far fewer cache misses).
-    if (prog.n_states == 1 && opt.coef_budget > 0) {
+    // Only sweeps of many tiles are split: with few workgroups in flight
+    // the cache holds their coefficients anyway.
+    if (prog.n_states == 1 && opt.coef_budget > 0 && sw.n_tiles >= 512) {
         auto phase_bytes = [&](const PhaseInfo& ph) {
             long b = 0;
             for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
