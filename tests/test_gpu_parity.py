@@ -99,7 +99,7 @@ def test_u3_brickwall_and_qft_vs_oracle(n):
     g = W.u3_brickwall(n=n, layers=20)
     want = O.run(n, g)
     assert _err(_gpu_run(n, g), want) < TOL
-    assert _err(_gpu_run(n, g, coef=6000), want) < TOL        # sweeps split by coefficient bytes
+    assert _err(_gpu_run(n, g, coef=3000, tile=8), want) < TOL  # >= 512 tiles: sweeps split by coefficient bytes
     assert _err(_gpu_run(n, g, pipeline=1), want) < TOL       # persistent prefetching sweeps
     st = W.random_state(n, seed=3000 + n)
     q = W.qft(n=n)
