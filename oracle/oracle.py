@@ -16,12 +16,24 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libsvsim_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libsvsim_ref.so")
+REF_QIR_SO = os.path.join(HERE, "_ref", "libsvsim_qir_ref.so")
 
 # Gate record layout shared by the oracle, the reference driver and svmi.h:
 # {int32 op; int32 qb[5]; double theta, phi, lambda;}  -> 48 bytes.
 GATE_DTYPE = np.dtype([("op", "<i4"), ("qb", "<i4", (5,)),
                        ("theta", "<f8"), ("phi", "<f8"), ("lambda", "<f8")])
 PAULI_DTYPE = np.dtype([("coeff", "<f8"), ("xmask", "<u8"), ("zmask", "<u8")])
+# QIR gate record (svsim_qsharp/svsim_cpu_omp.hpp:113-147; svmi_qir_gate):
+# {int32 op; int32 qubit; uint64 mask; double theta;} -> 24 bytes.
+QIR_GATE_DTYPE = np.dtype([("op", "<i4"), ("qubit", "<i4"), ("mask", "<u8"), ("theta", "<f8")])
+
+# QIR op numbering (svsim_qsharp/svsim_cpu_omp.hpp:45-59).
+QIR_OPS = ["X", "Y", "Z", "H", "S", "T", "RI", "RX", "RY", "RZ", "EI", "EX", "EY", "EZ",
+           "ControlledX", "ControlledY", "ControlledZ", "ControlledH", "ControlledS", "ControlledT",
+           "ControlledRI", "ControlledRX", "ControlledRY", "ControlledRZ",
+           "ControlledEI", "ControlledEX", "ControlledEY", "ControlledEZ",
+           "AdjointS", "AdjointT", "ControlledAdjointS", "ControlledAdjointT", "Swap", "Measure"]
+QIR = {name: i for i, name in enumerate(QIR_OPS)}
 
 # libc RAND_MAX on glibc; the reference draws r = rand()/RAND_MAX (cpu_omp:351).
 RAND_MAX = 2147483647
@@ -64,12 +76,18 @@ def lib():
         L.orc_apply_pauli_sum.restype = None
         L.orc_gate_arity.argtypes = [ctypes.c_int]
         L.orc_gate_arity.restype = ctypes.c_int
+        L.orc_qir_apply.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, ctypes.c_int]
+        L.orc_qir_apply.restype = ctypes.c_int
         _lib = L
     return _lib
 
 
 def ref_available():
     return os.path.exists(REF_SO)
+
+
+def ref_qir_available():
+    return os.path.exists(REF_QIR_SO)
 
 
 def ref():
@@ -230,3 +248,67 @@ def ref_measure(n, re, im, draws):
     if rc != 0:
         raise ValueError("reference measure failed")
     return out
+
+
+# ---- QIR gate set: multi-controlled gates and measure-and-collapse ----
+
+def as_qir_gates(gates):
+    """Accept a QIR_GATE_DTYPE array or a list of (op, qubit, mask, theta)."""
+    if isinstance(gates, np.ndarray) and gates.dtype == QIR_GATE_DTYPE:
+        return np.ascontiguousarray(gates)
+    arr = np.zeros(len(gates), dtype=QIR_GATE_DTYPE)
+    for i, g in enumerate(gates):
+        op = QIR[g[0]] if isinstance(g[0], str) else int(g[0])
+        arr[i] = (op, int(g[1]), int(g[2]) if len(g) > 2 else 0, float(g[3]) if len(g) > 3 else 0.0)
+    return arr
+
+
+def n_measures(gates):
+    return int(np.count_nonzero(as_qir_gates(gates)["op"] == QIR["Measure"]))
+
+
+def qir_apply(n, re, im, gates, last_prob=0.0, nthreads=0):
+    """Apply QIR gates to the first 2^n amplitudes of (re, im) in place with
+    the oracle.  Returns (probs, outcomes, last_prob) for the Measure records."""
+    g = as_qir_gates(gates)
+    assert re.dtype == np.float64 and im.dtype == np.float64 and re.flags.c_contiguous and im.flags.c_contiguous
+    assert len(re) >= (1 << n) and len(im) >= (1 << n)
+    nm = int(np.count_nonzero(g["op"] == QIR["Measure"]))
+    probs = np.zeros(max(nm, 1))
+    outs = np.zeros(max(nm, 1), dtype=np.int32)
+    lp = np.array([last_prob], dtype=np.float64)
+    rc = lib().orc_qir_apply(n, _ptr(re), _ptr(im), _ptr(g), len(g), _ptr(lp), _ptr(probs), _ptr(outs),
+                             int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle rejected the QIR gate list (bad op, target or swap qubit)")
+    return probs[:nm], outs[:nm], float(lp[0])
+
+
+def ref_qir_run(capacity, n, gates, re, im, nthreads=1):
+    """The reference QIR engine (oracle/_ref/libsvsim_qir_ref.so) on a copy of
+    the 2^capacity-amplitude state with n allocated qubits.
+    Returns (re, im, probs, outcomes)."""
+    global _ref_qir
+    if _ref_qir is None:
+        if not ref_qir_available():
+            raise RuntimeError("oracle/_ref QIR engine not built (reference sources absent)")
+        L = ctypes.CDLL(REF_QIR_SO)
+        L.ref_qir_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_size_t, _vp, _vp,
+                                  _vp, _vp]
+        L.ref_qir_run.restype = ctypes.c_int
+        _ref_qir = L
+    g = as_qir_gates(gates)
+    ore = np.array(re, dtype=np.float64, copy=True)
+    oim = np.array(im, dtype=np.float64, copy=True)
+    assert len(ore) == 1 << capacity
+    nm = int(np.count_nonzero(g["op"] == QIR["Measure"]))
+    probs = np.zeros(max(nm, 1))
+    outs = np.zeros(max(nm, 1), dtype=np.int32)
+    rc = _ref_qir.ref_qir_run(capacity, n, nthreads, _ptr(g), len(g), _ptr(ore), _ptr(oim), _ptr(probs),
+                              _ptr(outs))
+    if rc != 0:
+        raise ValueError("reference QIR driver rejected the gate list")
+    return ore, oim, probs[:nm], outs[:nm]
+
+
+_ref_qir = None
