@@ -178,6 +178,51 @@ typedef struct svmi_batch_item {
 svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t n_items,
                                double* expvals, double* grads /* concatenated n_params */);
 
+/* ---- QIR gate set: the reference's QIR-facing engine
+ *      (svsim_qsharp/svsim_cpu_omp.hpp), which its Q#/QIR runtime driver
+ *      (svsim_qsharp/svsim_qsharp_wrapper.cpp) feeds.  Same numbering as its
+ *      enum OP (svsim_cpu_omp.hpp:45-59). */
+typedef enum svmi_qir_op {
+    SVMI_QIR_X, SVMI_QIR_Y, SVMI_QIR_Z, SVMI_QIR_H, SVMI_QIR_S, SVMI_QIR_T,
+    SVMI_QIR_RI, SVMI_QIR_RX, SVMI_QIR_RY, SVMI_QIR_RZ, SVMI_QIR_EI, SVMI_QIR_EX, SVMI_QIR_EY, SVMI_QIR_EZ,
+    SVMI_QIR_CONTROLLED_X, SVMI_QIR_CONTROLLED_Y, SVMI_QIR_CONTROLLED_Z,
+    SVMI_QIR_CONTROLLED_H, SVMI_QIR_CONTROLLED_S, SVMI_QIR_CONTROLLED_T,
+    SVMI_QIR_CONTROLLED_RI, SVMI_QIR_CONTROLLED_RX, SVMI_QIR_CONTROLLED_RY, SVMI_QIR_CONTROLLED_RZ,
+    SVMI_QIR_CONTROLLED_EI, SVMI_QIR_CONTROLLED_EX, SVMI_QIR_CONTROLLED_EY, SVMI_QIR_CONTROLLED_EZ,
+    SVMI_QIR_ADJOINT_S, SVMI_QIR_ADJOINT_T, SVMI_QIR_CONTROLLED_ADJOINT_S, SVMI_QIR_CONTROLLED_ADJOINT_T,
+    SVMI_QIR_SWAP, SVMI_QIR_MEASURE,
+    SVMI_QIR_NUM_OPS
+} svmi_qir_op;
+
+/* The reference's QIR Gate record (svsim_cpu_omp.hpp:113-147) without its
+ * function pointer.  Controlled ops: `mask` holds the control qubits as bits
+ * (the op acts where all of them are 1; a mask naming the target or a qubit
+ * >= n_qubits makes it the identity).  Swap: `mask` is the second qubit.
+ * Measure: `theta` is the uniform draw and `mask` the basis (0 Z, 1 X, 2 Y). */
+typedef struct svmi_qir_gate {
+    int32_t op;
+    int32_t qubit;
+    uint64_t mask;
+    double theta;
+} svmi_qir_gate;
+
+/* ---- Simulation::sim of the QIR engine (svsim_cpu_omp.hpp:467-505) with
+ *      mid-circuit measurement (Measure_GATE, 1406-1500).  Applies the gates
+ *      in order to the first 2^n_qubits amplitudes of the device state
+ *      (n_qubits = the allocated qubits, 1 <= n_qubits <= the handle's
+ *      qubits; with n_qubits < 4 the amplitudes at index >= 2^n_qubits must
+ *      be zero, as they are after qubits are released in |0>).  A Measure
+ *      record computes P(1) of its qubit in its basis, then collapses: if
+ *      draw < P(1) the bit-0 amplitudes are zeroed and the bit-1 ones scaled
+ *      by 1/sqrt(P(1)), otherwise the converse with 1/sqrt(1 - P(1)).
+ *      probs[m] / outcomes[m] (optional, one per Measure record) receive
+ *      P(1) and (draw < P(1)); *last_prob (optional) receives the P(1) of
+ *      the latest Measure run on this handle, 0 before any (the value
+ *      Simulation::sim returns).  Every decision stays on the device: the
+ *      call synchronises once, at its end. */
+svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates,
+                         double* probs, int32_t* outcomes, double* last_prob);
+
 /* ---- options, stats, errors */
 svmi_status svmi_set_option(svmi_handle h, int option, int64_t value);
 svmi_status svmi_get_stats(svmi_handle h, svmi_stats* out);

@@ -47,6 +47,16 @@ svmi_status svmi_debug_plan_adjoint(int n, const svmi_gate* gates, size_t n_gate
                                     size_t n_params, int tile_qubits, int low_qubits, uint64_t* n_passes,
                                     char* buf, size_t buflen);
 
+/* Host execution of svmi_qir_run's program (n <= 26 qubits, state of 2^n
+ * amplitudes in and out; probs / outcomes optional, one per Measure). */
+svmi_status svmi_debug_emulate_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits,
+                                   int low_qubits, double* re, double* im, double* probs, int32_t* outcomes);
+
+/* Plan a QIR gate list as svmi_qir_run does: sweeps, measurement steps and
+ * a text description (truncated to buflen-1 bytes; buf may be NULL). */
+svmi_status svmi_debug_plan_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
+                                uint64_t* n_sweeps, uint64_t* n_measures, char* buf, size_t buflen);
+
 /* Planner summary: number of full-state passes and a text description
  * (truncated to buflen-1 bytes; buf may be NULL). */
 svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates,

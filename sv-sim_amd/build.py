@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-ENGINE_SOURCES = ["svmi_api.hip", "svmi_kernels.hip", "svmi_lower.cpp", "svmi_plan.cpp"]
+ENGINE_SOURCES = ["svmi_api.hip", "svmi_kernels.hip", "svmi_lower.cpp", "svmi_plan.cpp", "svmi_qir.cpp"]
 LIB = os.path.join(HERE, "libsvmi.so")
 
 

@@ -16,6 +16,7 @@
 #include "svmi_kernels.hpp"
 #include "svmi_lower.hpp"
 #include "svmi_plan.hpp"
+#include "svmi_qir.hpp"
 
 using namespace svmi;
 
@@ -127,6 +128,33 @@ void plan_forward(const svmi_gate* gates, size_t n_gates, int n, const PlanOptio
     } else {
         build_forward_program(low, n, opt, prog);
     }
+}
+
+// QIR gate list on n allocated qubits -> segmented program on n_prog >= n
+// qubits (n_prog >= 4: every sweep needs 4 register bits).
+svmi_status plan_qir(const svmi_qir_gate* gates, size_t n_gates, int n, int n_prog, const PlanOptions& opt,
+                     Program& prog)
+{
+    if (n_gates && !gates) return fail(SVMI_ERR_ARG, "null gate array");
+    std::string err;
+    for (size_t i = 0; i < n_gates; i++) {
+        const svmi_status st = validate_qir_gate(gates[i], n, err);
+        if (st != SVMI_OK) return fail(st, "QIR gate " + std::to_string(i) + ": " + err);
+    }
+    std::vector<std::vector<LOp>> segs;
+    std::vector<MeasureStep> meas;
+    lower_qir(gates, n_gates, n, segs, meas);
+    for (auto& seg : segs) {
+        std::vector<LOp> f;
+        fuse_single_qubit_runs(seg, f);
+        seg.swap(f);
+    }
+    try {
+        build_segmented_program(segs, meas, n_prog, opt, prog);
+    } catch (const std::exception& e) {
+        return fail(SVMI_ERR_ARG, std::string("QIR planning: ") + e.what());
+    }
+    return SVMI_OK;
 }
 
 svmi_status check_terms(const svmi_pauli_term* terms, size_t n_terms, int n)
@@ -567,6 +595,12 @@ struct svmi_sim {
     Program adj_prog;
     DeviceProgram adj_dprog;
     DevBuf adj_gsum;
+    // QIR runs: program, device copy, measurement results; the P(1) of the
+    // latest measurement (0 after create / reset_state)
+    Program qir_prog;
+    DeviceProgram qir_dprog;
+    DevBuf qir_meas;
+    double qir_last_prob = 0.0;
 };
 
 namespace {
@@ -591,7 +625,8 @@ svmi_status use_device(svmi_handle h)
 }
 
 svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, c64* s0, c64* s1,
-                         double* grad_sums /* device, prog.n_grad */)
+                         double* grad_sums /* device, prog.n_grad */,
+                         MeasureResult* meas = nullptr /* device, prog.measures.size() */)
 {
     const bool prof = h->profile;
     if (prof) {
@@ -616,6 +651,11 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
                                   h->pipeline));
             if (partials)
                 SVMI_HIP(launch_sum_rows(h->stream, partials, sw.n_grad, sw.n_tiles, grad_sums + sw.grad_base));
+        } else if (st.kind == Step::MEASURE) {
+            const MeasureStep& m = prog.measures[st.index];
+            SVMI_TRY(h->scratch[4].ensure(measure_partials(int(m.n)) * sizeof(double)));
+            SVMI_HIP(launch_measure(h->stream, s0, int(m.n), m.qubit, m.draw, h->scratch[4].as<double>(),
+                                    dp.coef.as<c64>() + m.coef, meas + st.index));
         } else {
             const StandaloneOp& so = prog.standalone[st.index];
             SVMI_HIP(launch_standalone(h->stream, s0, h->n_int, so, dp.coef.as<c64>()));
@@ -739,6 +779,7 @@ int svmi_num_qubits(svmi_handle h) { return h ? h->n : -1; }
 svmi_status svmi_reset_state(svmi_handle h)
 {
     SVMI_TRY(use_device(h));
+    h->qir_last_prob = 0.0;   // reset() also clears m_real (svsim_qsharp/svsim_cpu_omp.hpp:430-438)
     SVMI_HIP(launch_fill_basis(h->stream, h->sv, h->dim));
     SVMI_HIP(hipStreamSynchronize(h->stream));
     return SVMI_OK;
@@ -874,6 +915,43 @@ svmi_status svmi_run(svmi_handle h)
     float ms = 0.f;
     SVMI_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.device_ms = ms;
+    return SVMI_OK;
+}
+
+svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates, double* probs,
+                         int32_t* outcomes, double* last_prob)
+{
+    SVMI_TRY(use_device(h));
+    if (n_qubits < 1 || n_qubits > h->n)
+        return fail(SVMI_ERR_ARG, "n_qubits must be in [1, " + std::to_string(h->n) + "] (the handle's qubits)");
+    SVMI_TRY(plan_qir(gates, n_gates, n_qubits, std::max(n_qubits, kRegBits), h->opt, h->qir_prog));
+    const Program& prog = h->qir_prog;
+    SVMI_TRY(h->qir_dprog.upload(prog, h->stream));
+    const size_t nm = prog.measures.size();
+    SVMI_TRY(h->qir_meas.ensure(std::max<size_t>(nm, 1) * sizeof(MeasureResult)));
+    h->stats.sweeps = 0;
+    h->stats.kernel_launches = 0;
+    h->stats.sweep_launches = h->stats.other_launches = 0;
+    h->stats.sweep_ms = h->stats.other_ms = 0.0;
+    h->stats.gates = n_gates;
+    h->stats.ops = prog.n_lowered;
+    SVMI_HIP(hipEventRecord(h->ev0, h->stream));
+    SVMI_TRY(exec_program(h, prog, h->qir_dprog, h->sv, nullptr, nullptr, h->qir_meas.as<MeasureResult>()));
+    SVMI_HIP(hipEventRecord(h->ev1, h->stream));
+    std::vector<MeasureResult> res(nm);
+    if (nm)
+        SVMI_HIP(hipMemcpyAsync(res.data(), h->qir_meas.p, nm * sizeof(MeasureResult), hipMemcpyDeviceToHost,
+                                h->stream));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    float ms = 0.f;
+    SVMI_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->stats.device_ms = ms;
+    for (size_t m = 0; m < nm; m++) {
+        if (probs) probs[m] = res[m].prob;
+        if (outcomes) outcomes[m] = res[m].outcome;
+    }
+    if (nm) h->qir_last_prob = res[nm - 1].prob;
+    if (last_prob) *last_prob = h->qir_last_prob;
     return SVMI_OK;
 }
 
@@ -1253,11 +1331,36 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
     }
 }
 
-void host_program(const Program& prog, std::vector<c64>& s0, std::vector<c64>& s1, std::vector<double>& gsum)
+// A measurement step on the host: P(1) over the first 2^m.n amplitudes, the
+// outcome and the collapse factors written into the program's coefficients
+// (what k_prob_one / k_measure_decide do on the device).
+MeasureResult host_measure(const MeasureStep& m, const std::vector<c64>& s, std::vector<c64>& coef)
+{
+    double p = 0.0;
+    for (uint64_t x = 0; x < (1ull << m.n); x++)
+        if ((x >> m.qubit) & 1) p += s[x].re * s[x].re + s[x].im * s[x].im;
+    const bool one = m.draw < p;
+    const double f = one ? 1.0 / std::sqrt(p) : 1.0 / std::sqrt(1.0 - p);
+    coef[m.coef] = c64{one ? 0.0 : f, 0.0};
+    coef[m.coef + 1] = c64{one ? f : 0.0, 0.0};
+    MeasureResult r;
+    r.prob = p;
+    r.outcome = one ? 1 : 0;
+    r.pad = 0;
+    return r;
+}
+
+void host_program(Program& prog, std::vector<c64>& s0, std::vector<c64>& s1, std::vector<double>& gsum,
+                  std::vector<MeasureResult>* meas = nullptr)
 {
     const int ns = prog.n_states;
     gsum.assign(prog.n_grad, 0.0);
     for (const Step& st : prog.steps) {
+        if (st.kind == Step::MEASURE) {
+            const MeasureResult r = host_measure(prog.measures[st.index], s0, prog.coef);
+            if (meas) meas->push_back(r);
+            continue;
+        }
         if (st.kind == Step::STANDALONE) {
             host_standalone(s0, prog.n, prog.standalone[st.index], prog.coef);
             if (ns == 2) host_standalone(s1, prog.n, prog.standalone[st.index], prog.coef);
@@ -1456,6 +1559,49 @@ svmi_status svmi_debug_plan_batch(const svmi_batch_item* items, size_t n_items, 
     if (n_launches) *n_launches = bp.launches.size();
     if (n_tasks) *n_tasks = bp.n_tasks;
     if (state_bytes) *state_bytes = bp.state_bytes;
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_emulate_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits,
+                                   int low_qubits, double* re, double* im, double* probs, int32_t* outcomes)
+{
+    if (n < 1 || n > 26) return fail(SVMI_ERR_ARG, "emulation supports 1..26 qubits");
+    if (!re || !im) return fail(SVMI_ERR_ARG, "null state");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    const int ni = std::max(n, kRegBits);
+    Program prog;
+    SVMI_TRY(plan_qir(gates, n_gates, n, ni, debug_options(1, tile_qubits, low_qubits, 0), prog));
+    std::vector<c64> s(1ull << ni, c64{0.0, 0.0}), dummy;
+    for (uint64_t x = 0; x < (1ull << n); x++) s[x] = c64{re[x], im[x]};
+    std::vector<double> g;
+    std::vector<MeasureResult> res;
+    host_program(prog, s, dummy, g, &res);
+    for (uint64_t x = 0; x < (1ull << n); x++) {
+        re[x] = s[x].re;
+        im[x] = s[x].im;
+    }
+    for (size_t m = 0; m < res.size(); m++) {
+        if (probs) probs[m] = res[m].prob;
+        if (outcomes) outcomes[m] = res[m].outcome;
+    }
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_plan_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
+                                uint64_t* n_sweeps, uint64_t* n_measures, char* buf, size_t buflen)
+{
+    if (n < 1 || n > kMaxQubits) return fail(SVMI_ERR_ARG, "bad n");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    Program prog;
+    SVMI_TRY(plan_qir(gates, n_gates, n, std::max(n, kRegBits), debug_options(1, tile_qubits, low_qubits, 0), prog));
+    if (n_sweeps) *n_sweeps = prog.sweeps.size();
+    if (n_measures) *n_measures = prog.measures.size();
+    if (buf && buflen) {
+        const std::string d = describe_program(prog);
+        const size_t m = std::min(buflen - 1, d.size());
+        std::memcpy(buf, d.data(), m);
+        buf[m] = 0;
+    }
     return SVMI_OK;
 }
 

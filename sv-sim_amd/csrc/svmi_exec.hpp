@@ -112,6 +112,21 @@ SVMI_HD inline void dispatch_srn(int s, c64 (&a)[kRegs])
     }
 }
 
+SVMI_HD inline void dispatch_cdense(const DOpHdr& h, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+                                    uint64_t base)
+{
+    if ((base & h.emask) != h.emask) return;
+    if ((goff & h.gmask) != h.gmask) return;
+    const c64* M = coef + h.coef;
+    switch (h.s0) {
+        SVMI_CASE_ID(0, 200, apply_cdense<0>(a, M, h.rsel))
+        SVMI_CASE_ID(1, 201, apply_cdense<1>(a, M, h.rsel))
+        SVMI_CASE_ID(2, 202, apply_cdense<2>(a, M, h.rsel))
+        SVMI_CASE_ID(3, 203, apply_cdense<3>(a, M, h.rsel))
+    default: break;
+    }
+}
+
 // Index of a diagonal op's table entry for tile-local index l.
 // (fixed trip count so every index stays static after unrolling)
 SVMI_HD inline uint32_t diag_index(const DOp& op, uint32_t l, uint32_t eidx)
@@ -223,6 +238,7 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_CASE_ID(DOP_CPHASE, DOP_CPHASE + 100, apply_cphase(h, a, coef, goff, base))
         SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag(*op, a, coef, goff, off, base))
         SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset(h, a, coef, goff, base, xf))
+        SVMI_CASE_ID(DOP_CDENSE, DOP_CDENSE + 100, dispatch_cdense(h, a, coef, goff, base))
     default: break;
     }
 }

@@ -77,6 +77,9 @@ enum DOpType : uint16_t {
     DOP_GRAD_DENSE = 8,  // adjoint: acc[grad] += Re sum conj(lam) (coef-matrix phi) on slots
     DOP_GRAD_DIAG = 9,   // adjoint: acc[grad] += Re sum conj(lam) coef[idx] phi
     DOP_PHASESET = 10,   // product of controlled phases, applied to registers rsel
+    DOP_CDENSE = 11,     // 2x2 coef on register slot s0, on the register pairs in rsel
+                         // where the group bits gmask and the bits emask outside the
+                         // tile are all 1 (multi-controlled single-qubit gate)
 };
 
 // One controlled phase of a DOP_PHASESET (2 c64 slots in the coefficient
@@ -107,9 +110,9 @@ struct DOpHdr {
     uint32_t s0, s1;     // X/CX/SWAP/SRN slots
     uint32_t coef;       // offset into the coefficient array (c64 units)
     uint32_t grad;       // GRAD_*: accumulator slot within the sweep
-    uint32_t rsel;       // CPHASE: registers rho whose register bits satisfy the op
-    uint32_t gmask;      // CPHASE: non-register tile-local bits that must be 1
-    uint64_t emask;      // CPHASE: physical bits outside the tile that must be 1
+    uint32_t rsel;       // CPHASE/DIAG/CDENSE: registers rho whose register bits satisfy the op
+    uint32_t gmask;      // CPHASE/DIAG/CDENSE: non-register tile-local bits that must be 1
+    uint64_t emask;      // CPHASE/DIAG/CDENSE: physical bits outside the tile that must be 1
 };
 struct DOp {
     DOpHdr h;
@@ -269,6 +272,22 @@ SVMI_HD inline void apply_swap(c64 (&a)[kRegs])
             const c64 tmp = a[r];
             a[r] = a[r2];
             a[r2] = tmp;
+        }
+}
+
+// Controlled 2x2 (row-major M) on register slot s: the pairs (r, r | 1<<s)
+// whose register r is in rsel (the register part of the control condition;
+// rsel never tells r from r | 1<<s apart).
+template <int s>
+SVMI_HD inline void apply_cdense(c64 (&a)[kRegs], const c64* __restrict__ M, uint32_t rsel)
+{
+    const c64 m00 = M[0], m01 = M[1], m10 = M[2], m11 = M[3];
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (!((r >> s) & 1) && ((rsel >> r) & 1u)) {
+            const c64 a0 = a[r], a1 = a[r | (1 << s)];
+            a[r] = cmac(cmac(c64{0.0, 0.0}, m00, a0), m01, a1);
+            a[r | (1 << s)] = cmac(cmac(c64{0.0, 0.0}, m10, a0), m11, a1);
         }
 }
 

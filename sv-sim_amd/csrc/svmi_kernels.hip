@@ -1022,6 +1022,88 @@ k_sum_rows(const double* __restrict__ src, uint64_t len, double* __restrict__ ds
     if (threadIdx.x == 0) dst[blockIdx.x] = red[0];
 }
 
+// ------------------------------------------------------- measurement
+//
+// P(1) of qubit q: thread t of the grid reads the amplitudes with bit q set
+// at pair indices i = t, t + G, ... (G = grid threads; pos1 = i with a 1
+// inserted at bit q), four loads in flight, and sums |a|^2 in that order;
+// then a fixed tree per workgroup, and one workgroup sums the partials in a
+// fixed order.  The same order on every run.
+
+__global__ void __launch_bounds__(kThreads)
+k_prob_one(const c64* __restrict__ s, uint64_t half, uint32_t q, double* __restrict__ partials)
+{
+    __shared__ double red[kThreads];
+    const uint64_t G = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t lo = (1ull << q) - 1;
+    double acc = 0.0;
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i + 3 * G < half; i += 4 * G) {
+        c64 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint64_t j = i + u * G;
+            v[u] = s[((j & ~lo) << 1) | (1ull << q) | (j & lo)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc += v[u].re * v[u].re + v[u].im * v[u].im;
+    }
+    for (; i < half; i += G) {
+        const c64 v = s[((i & ~lo) << 1) | (1ull << q) | (i & lo)];
+        acc += v.re * v.re + v.im * v.im;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_measure_decide(const double* __restrict__ partials, uint32_t nparts, double draw, c64* __restrict__ collapse,
+                 MeasureResult* __restrict__ res)
+{
+    __shared__ double red[kThreads];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) acc += partials[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double p = red[0];
+        const bool one = draw < p;
+        const double f = one ? 1.0 / sqrt(p) : 1.0 / sqrt(1.0 - p);
+        collapse[0] = c64{one ? 0.0 : f, 0.0};
+        collapse[1] = c64{one ? f : 0.0, 0.0};
+        MeasureResult r;
+        r.prob = p;
+        r.outcome = one ? 1 : 0;
+        r.pad = 0;
+        *res = r;
+    }
+}
+
+uint32_t measure_partials(int n)
+{
+    const uint64_t half = 1ull << (n - 1);
+    const uint64_t g = (half + kThreads * 4 - 1) / (kThreads * 4);
+    return uint32_t(g > 2048 ? 2048 : (g < 1 ? 1 : g));
+}
+
+hipError_t launch_measure(hipStream_t st, const c64* s, int n, uint32_t q, double draw, double* partials,
+                          c64* collapse, MeasureResult* res)
+{
+    const uint32_t g = measure_partials(n);
+    hipLaunchKernelGGL(k_prob_one, dim3(g), dim3(kThreads), 0, st, s, 1ull << (n - 1), q, partials);
+    hipLaunchKernelGGL(k_measure_decide, dim3(1), dim3(kThreads), 0, st, partials, g, draw, collapse, res);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_segments(hipStream_t st, const double* src, const SumSegment* segs, uint64_t nseg,
                                double* dst)
 {

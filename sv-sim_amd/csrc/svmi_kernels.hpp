@@ -105,4 +105,18 @@ hipError_t launch_sum_segments(hipStream_t st, const double* src, const SumSegme
 // dst[r] = sum of src[r * len, (r + 1) * len), r < rows, same order.
 hipError_t launch_sum_rows(hipStream_t st, const double* src, uint32_t rows, uint64_t len, double* dst);
 
+// One mid-circuit measurement on the device.
+struct MeasureResult {
+    double prob;         // P(1)
+    int32_t outcome;     // draw < P(1)
+    int32_t pad;
+};
+// P(1) of qubit q over the 2^n-amplitude state s, the outcome for `draw`,
+// and the collapse factors written into collapse[0] (bit 0) and collapse[1]
+// (bit 1): (0, 1/sqrt(P)) for outcome 1, (1/sqrt(1-P), 0) for outcome 0.
+// partials: measure_partials(n) doubles of scratch.
+uint32_t measure_partials(int n);
+hipError_t launch_measure(hipStream_t st, const c64* s, int n, uint32_t q, double draw, double* partials,
+                          c64* collapse, MeasureResult* res);
+
 }  // namespace svmi

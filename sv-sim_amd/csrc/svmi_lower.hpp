@@ -18,6 +18,14 @@ struct LOp {
     int q[5] = {0, 0, 0, 0, 0};
     std::vector<c64> m;
     int gate = -1;        // source gate index
+    // Control qubits (bit mask, disjoint from q): the op acts only on the
+    // amplitudes whose index has every bit of ctrl set (the identity
+    // elsewhere).  Controlled DENSE ops have k == 1.
+    uint64_t ctrl = 0;
+    // >= 0: a DIAG op (k == 1) whose two entries are written on the device
+    // by measurement `dyn` of the program (the collapse factors); m holds
+    // placeholders and the op is never merged, folded or dropped.
+    int dyn = -1;
 };
 
 // Number of qubit slots used by a reference op (1..5), -1 if unknown.

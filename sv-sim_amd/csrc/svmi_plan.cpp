@@ -15,6 +15,7 @@
 #include <deque>
 #include <cstring>
 #include <sstream>
+#include <stdexcept>
 
 namespace svmi {
 
@@ -79,10 +80,11 @@ SOp make_sop(const LOp* op, bool grad, int param, int max_q = kRegBits)
     s.mask = op_mask(*op);
     // The reference SRN (svsim_cpu_omp.hpp:866-879) is only real-linear: it
     // does not commute with complex phases even on other qubits.
-    s.dep = op->kind == LOp::SRN ? ~uint64_t(0) : s.mask;
+    // A controlled op must also stay ordered with every op on its controls.
+    s.dep = op->kind == LOp::SRN ? ~uint64_t(0) : (s.mask | op->ctrl);
     s.diag = op->kind == LOp::DIAG;
     s.standalone = op->kind == LOp::DENSE && op->k > max_q && !grad;
-    s.identity = !grad && is_identity(*op);
+    s.identity = !grad && op->dyn < 0 && is_identity(*op);
     return s;
 }
 
@@ -119,7 +121,7 @@ bool is_register_perm(const LOp& op)
 bool is_cphase(const SOp& s)
 {
     const LOp& op = *s.op;
-    if (op.kind != LOp::DIAG) return false;
+    if (op.kind != LOp::DIAG || op.dyn >= 0) return false;
     const size_t D = size_t(1) << op.k;
     for (size_t r = 0; r + 1 < D; r++)
         if (op.m[r].re != 1.0 || op.m[r].im != 0.0) return false;
@@ -142,12 +144,7 @@ struct Encoder {
             uint32_t rbits = 0;
             PhaseEntry e;
             std::memset(&e, 0, sizeof(e));
-            for (int q = 0; q < op.k; q++) {
-                const int l = tloc[op.q[q]];
-                if (l < 0) e.emask |= uint64_t(1) << op.q[q];
-                else if (slot_of[l] >= 0) rbits |= 1u << slot_of[l];
-                else e.gmask |= 1u << l;
-            }
+            cond_bits(op_mask(op) | op.ctrl, tloc, slot_of, rbits, e.gmask, e.emask);
             const c64 ph = op.m[(size_t(1) << op.k) - 1];
             e.ph_re = ph.re;
             e.ph_im = ph.im;
@@ -188,6 +185,28 @@ struct Encoder {
         }
     }
 
+    // Split the qubits of a condition ("all these bits are 1") into register
+    // slots, other tile-local bits and bits outside the tile.
+    static void cond_bits(uint64_t bits, const int* tloc, const int* slot_of, uint32_t& rbits, uint32_t& gmask,
+                          uint64_t& emask)
+    {
+        for (; bits; bits &= bits - 1) {
+            const int p = __builtin_ctzll(bits);
+            const int l = tloc[p];
+            if (l < 0) emask |= uint64_t(1) << p;
+            else if (slot_of[l] >= 0) rbits |= 1u << slot_of[l];
+            else gmask |= 1u << l;
+        }
+    }
+
+    static uint32_t rsel_of(uint32_t rbits)
+    {
+        uint32_t rsel = 0;
+        for (int r = 0; r < kRegs; r++)
+            if ((uint32_t(r) & rbits) == rbits) rsel |= 1u << r;
+        return rsel;
+    }
+
     uint32_t push_coef(const std::vector<c64>& v)
     {
         const uint32_t off = uint32_t(prog.coef.size());
@@ -212,28 +231,41 @@ struct Encoder {
         }
         if (op.kind == LOp::DIAG) {
             const size_t D = size_t(1) << op.k;
-            bool cphase = true;
+            bool cphase = op.dyn < 0;
             for (size_t r = 0; r + 1 < D; r++)
                 if (op.m[r].re != 1.0 || op.m[r].im != 0.0) cphase = false;
+            // only controlled phases carry controls (the QIR lowering splits
+            // a controlled diagonal into two of them)
+            if (op.ctrl && (!cphase || s.grad)) throw std::logic_error("controlled diagonal that is not a phase");
             d.nq = uint32_t(op.k);
-            uint32_t rbits = 0;   // register slots the op needs set
             for (int i = 0; i < op.k; i++) {
-                const int l = tloc[op.q[i]];
-                d.qloc[i] = int32_t(l);
+                d.qloc[i] = int32_t(tloc[op.q[i]]);
                 d.qphys[i] = uint32_t(op.q[i]);
-                if (l < 0) d.h.emask |= uint64_t(1) << op.q[i];
-                else if (slot_of[l] >= 0) rbits |= 1u << slot_of[l];
-                else d.h.gmask |= 1u << l;
             }
-            for (int r = 0; r < kRegs; r++)
-                if ((uint32_t(r) & rbits) == rbits) d.h.rsel |= 1u << r;
+            // CPHASE condition: every qubit of the op and every control is 1
+            uint32_t rbits = 0;
+            cond_bits(op_mask(op) | op.ctrl, tloc, slot_of, rbits, d.h.gmask, d.h.emask);
+            d.h.rsel = rsel_of(rbits);
             if (cphase && !s.grad) {
                 d.h.type = DOP_CPHASE;
                 d.h.coef = push_coef(std::vector<c64>{op.m[D - 1]});
             } else {
                 d.h.type = s.grad ? DOP_GRAD_DIAG : DOP_DIAG;
                 d.h.coef = push_coef(op.m);
+                if (op.dyn >= 0) prog.measures.at(size_t(op.dyn)).coef = d.h.coef;
             }
+            prog.ops.push_back(d);
+            return;
+        }
+        if (op.ctrl) {
+            // multi-controlled single-qubit gate
+            if (op.k != 1 || s.grad) throw std::logic_error("controlled dense op wider than one qubit");
+            uint32_t rbits = 0;
+            cond_bits(op.ctrl, tloc, slot_of, rbits, d.h.gmask, d.h.emask);
+            d.h.type = DOP_CDENSE;
+            d.h.s0 = uint32_t(slot_of[tloc[op.q[0]]]);
+            d.h.rsel = rsel_of(rbits);
+            d.h.coef = push_coef(op.m);
             prog.ops.push_back(d);
             return;
         }
@@ -364,7 +396,7 @@ struct RunMerger {
 
     bool reg_local(const SOp* s) const
     {
-        if (s->grad || s->op->kind == LOp::SRN) return false;
+        if (s->grad || s->op->kind == LOp::SRN || s->op->ctrl || s->op->dyn >= 0) return false;
         for (int i = 0; i < s->op->k; i++) {
             const int l = tloc[s->op->q[i]];
             if (l < 0 || slot_of[l] < 0) return false;
@@ -438,7 +470,7 @@ struct RunMerger {
             run_mask |= s->mask;
             return;
         }
-        if ((s->mask & run_mask) || s->op->kind == LOp::SRN) flush();
+        if ((s->dep & run_mask) || s->op->kind == LOp::SRN) flush();
         out.push_back(s);
     }
 };
@@ -667,7 +699,7 @@ void schedule(std::vector<SOp>& sops, int n, const PlanOptions& opt, Program& pr
     std::vector<const SOp*> pending;
     for (auto& s : sops)
         if (!s.identity) pending.push_back(&s);
-    prog.n_lowered = pending.size();
+    prog.n_lowered += pending.size();
     if (!fused) {
         for (const SOp* s : pending) {
             if (s->grad) {
@@ -721,8 +753,18 @@ void schedule(std::vector<SOp>& sops, int n, const PlanOptions& opt, Program& pr
         }
         for (; i < pending.size(); i++) rest.push_back(pending[i]);
         if (taken.empty()) {
-            // the first pending op cannot go in a tile
-            emit_standalone(prog, *pending.front());
+            // the first pending op cannot go in a tile that keeps the low
+            // qubits.  A controlled op (which has no full-state kernel) gets a
+            // sweep of its own around its target; anything else runs alone.
+            const SOp* s = pending.front();
+            if (s->op->ctrl && popc(s->mask) <= kk) {
+                uint64_t T1 = s->mask;
+                for (int p = 0; p < n && popc(T1) < kk; p++) T1 |= uint64_t(1) << p;
+                const int c1 = std::min(c, __builtin_ctzll(~T1));
+                emit_sweep(prog, std::vector<const SOp*>{s}, T1, n, c1, opt);
+            } else {
+                emit_standalone(prog, *s);
+            }
             pending.erase(pending.begin());
             continue;
         }
@@ -741,7 +783,7 @@ void fuse_single_qubit_runs(const std::vector<LOp>& in, std::vector<LOp>& out)
     out.reserve(in.size());
     std::vector<int> last(64, -1);
     for (const LOp& op : in) {
-        const bool mergeable = op.k == 1 && op.kind != LOp::SRN;
+        const bool mergeable = op.k == 1 && op.kind != LOp::SRN && op.ctrl == 0 && op.dyn < 0;
         if (mergeable && last[op.q[0]] >= 0) {
             LOp merged;
             lop_compose_same(out[last[op.q[0]]], op, merged);
@@ -751,6 +793,7 @@ void fuse_single_qubit_runs(const std::vector<LOp>& in, std::vector<LOp>& out)
         out.push_back(op);
         if (op.kind == LOp::SRN) std::fill(last.begin(), last.end(), -1);  // real-linear only
         for (int i = 0; i < op.k; i++) last[op.q[i]] = -1;
+        for (uint64_t c = op.ctrl; c; c &= c - 1) last[__builtin_ctzll(c)] = -1;
         if (mergeable) last[op.q[0]] = int(out.size()) - 1;
     }
 }
@@ -763,6 +806,28 @@ void build_forward_program(const std::vector<LOp>& ops, int n, const PlanOptions
     sops.reserve(ops.size());
     for (const LOp& op : ops) sops.push_back(make_sop(&op, false, -1));
     schedule(sops, n, opt, prog);
+}
+
+void build_segmented_program(const std::vector<std::vector<LOp>>& segments, const std::vector<MeasureStep>& measures,
+                             int n, const PlanOptions& opt, Program& prog)
+{
+    prog = Program();
+    prog.n = n;
+    prog.n_states = 1;
+    PlanOptions o = opt;
+    o.fusion = true;   // controlled ops run inside sweeps only
+    for (size_t i = 0; i < segments.size(); i++) {
+        std::vector<SOp> sops;
+        sops.reserve(segments[i].size());
+        for (const LOp& op : segments[i]) sops.push_back(make_sop(&op, false, -1));
+        schedule(sops, n, o, prog);
+        if (i < measures.size()) {
+            prog.steps.push_back(Step{Step::MEASURE, uint32_t(prog.measures.size())});
+            prog.measures.push_back(measures[i]);
+        }
+    }
+    for (const MeasureStep& m : prog.measures)
+        if (m.coef == UINT32_MAX) throw std::logic_error("measurement without a collapse op");
 }
 
 void build_adjoint_program(const std::vector<AdjEntry>& entries, int n, const PlanOptions& opt,
@@ -782,9 +847,11 @@ std::string describe_program(const Program& prog)
 {
     std::ostringstream ss;
     size_t n_sweeps = 0, n_stand = 0;
-    for (const Step& st : prog.steps) (st.kind == Step::SWEEP ? n_sweeps : n_stand)++;
+    size_t n_meas = 0;
+    for (const Step& st : prog.steps)
+        (st.kind == Step::SWEEP ? n_sweeps : st.kind == Step::MEASURE ? n_meas : n_stand)++;
     ss << "n=" << prog.n << " steps=" << prog.steps.size() << " sweeps=" << n_sweeps
-       << " standalone=" << n_stand << " phases=" << prog.phases.size() << " dops=" << prog.ops.size()
+       << " standalone=" << n_stand << " measure=" << n_meas << " phases=" << prog.phases.size() << " dops=" << prog.ops.size()
        << " lowered=" << prog.n_lowered << "\n";
     for (const Step& st : prog.steps) {
         if (st.kind == Step::SWEEP) {
@@ -806,7 +873,7 @@ std::string describe_program(const Program& prog)
                 ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3] << " :";
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                     const DOp& d = prog.ops[o];
-                    static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS"};
+                    static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS", "CD"};
                     ss << " " << nm[d.h.type];
                     if (d.h.type == DOP_PHASESET) {
                         ss << d.h.s0;
@@ -816,6 +883,8 @@ std::string describe_program(const Program& prog)
                 }
                 ss << "\n";
             }
+        } else if (st.kind == Step::MEASURE) {
+            ss << "  measure q=" << prog.measures[st.index].qubit << "\n";
         } else {
             const StandaloneOp& so = prog.standalone[st.index];
             ss << "  standalone kind=" << so.kind << " q=" << so.q << "\n";

@@ -20,8 +20,19 @@ struct PlanOptions {
 
 // One step of a program: a tiled sweep or a standalone full-state op.
 struct Step {
-    enum Kind { SWEEP = 0, STANDALONE = 1 } kind;
+    enum Kind { SWEEP = 0, STANDALONE = 1, MEASURE = 2 } kind;
     uint32_t index;
+};
+
+// A mid-circuit measurement: P(1) of `qubit` over the state, the outcome
+// draw < P(1), and the collapse factors written into coefficient slots
+// coef, coef + 1 (the DIAG op with LOp::dyn == this measurement's index,
+// which runs first in the next segment).
+struct MeasureStep {
+    uint32_t qubit = 0;
+    uint32_t coef = UINT32_MAX;
+    uint32_t n = 0;        // P(1) sums over the first 2^n amplitudes
+    double draw = 0.0;
 };
 
 // Host copy of a device program.
@@ -35,6 +46,7 @@ struct Program {
     std::vector<c64> coef;
     std::vector<uint64_t> hitab;
     std::vector<StandaloneOp> standalone;
+    std::vector<MeasureStep> measures;
     // adjoint: accumulator g of sweep s -> gradient parameter index
     std::vector<int32_t> grad_param;
     uint32_t n_grad = 0;
@@ -47,6 +59,13 @@ void fuse_single_qubit_runs(const std::vector<LOp>& in, std::vector<LOp>& out);
 
 // Forward program for n qubits.
 void build_forward_program(const std::vector<LOp>& ops, int n, const PlanOptions& opt, Program& prog);
+
+// Program of segments separated by measurements: segments[i] is scheduled
+// on its own (no op moves across a measurement), then measurement i runs if
+// i < measures.size().  segments[i + 1] must contain the collapse op of
+// measurement i (a DIAG op with dyn == i).
+void build_segmented_program(const std::vector<std::vector<LOp>>& segments, const std::vector<MeasureStep>& measures,
+                             int n, const PlanOptions& opt, Program& prog);
 
 // An entry of an adjoint (backward) schedule: the inverse of one gate applied
 // to phi and lambda, or a gradient accumulation <lam| dU |phi> between them.

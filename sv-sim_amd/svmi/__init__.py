@@ -29,6 +29,15 @@ ARITY.update({"CCX": 3, "CSWAP": 3, "RCCX": 3, "RC3X": 4, "C3X": 4, "C3SQRTX": 4
 GATE_DTYPE = np.dtype([("op", "<i4"), ("qb", "<i4", (5,)),
                        ("theta", "<f8"), ("phi", "<f8"), ("lambda", "<f8")])
 PAULI_DTYPE = np.dtype([("coeff", "<f8"), ("xmask", "<u8"), ("zmask", "<u8")])
+# svmi_qir_gate: the reference QIR Gate record (svsim_qsharp/svsim_cpu_omp.hpp:113-147).
+QIR_GATE_DTYPE = np.dtype([("op", "<i4"), ("qubit", "<i4"), ("mask", "<u8"), ("theta", "<f8")])
+# QIR op numbering (svsim_qsharp/svsim_cpu_omp.hpp:45-59).
+QIR_OP_NAMES = ["X", "Y", "Z", "H", "S", "T", "RI", "RX", "RY", "RZ", "EI", "EX", "EY", "EZ",
+                "ControlledX", "ControlledY", "ControlledZ", "ControlledH", "ControlledS", "ControlledT",
+                "ControlledRI", "ControlledRX", "ControlledRY", "ControlledRZ",
+                "ControlledEI", "ControlledEX", "ControlledEY", "ControlledEZ",
+                "AdjointS", "AdjointT", "ControlledAdjointS", "ControlledAdjointT", "Swap", "Measure"]
+QIR_OP = {name: i for i, name in enumerate(QIR_OP_NAMES)}
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
           5: "ERR_DEVICE", 6: "ERR_OOM", 7: "ERR_UNSUPPORTED"}
@@ -40,9 +49,9 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
 EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state", "svmi_set_random_state",
            "svmi_set_state", "svmi_get_state", "svmi_get_amplitudes", "svmi_upload", "svmi_clear_circuit",
            "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
-           "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version",
+           "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version", "svmi_qir_run",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
-           "svmi_debug_plan_batch", "svmi_debug_plan_adjoint"]
+           "svmi_debug_plan_batch", "svmi_debug_plan_adjoint", "svmi_debug_emulate_qir", "svmi_debug_plan_qir"]
 
 
 class SvmiError(RuntimeError):
@@ -117,6 +126,12 @@ def lib():
                                       ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
         L.svmi_debug_plan_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
+        L.svmi_qir_run.argtypes = [_vp, ctypes.c_int, _vp, _sz, _vp, _vp, _vp]
+        L.svmi_debug_emulate_qir.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                             _vp]
+        L.svmi_debug_plan_qir.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.c_char_p, _sz]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("svmi_num_qubits", "svmi_last_error", "svmi_abi_version"):
@@ -146,6 +161,18 @@ def as_gates(gates):
         arr[i]["qb"][: len(qs)] = qs
         vals = (list(g[2:5]) + [0.0, 0.0, 0.0])[:3]
         arr[i]["theta"], arr[i]["phi"], arr[i]["lambda"] = vals
+    return arr
+
+
+def as_qir_gates(gates):
+    """Accept a QIR_GATE_DTYPE array or a list of (op, qubit, mask, theta)
+    with op a name or number."""
+    if isinstance(gates, np.ndarray) and gates.dtype == QIR_GATE_DTYPE:
+        return np.ascontiguousarray(gates)
+    arr = np.zeros(len(gates), dtype=QIR_GATE_DTYPE)
+    for i, g in enumerate(gates):
+        op = QIR_OP[g[0]] if isinstance(g[0], str) else int(g[0])
+        arr[i] = (op, int(g[1]), int(g[2]) if len(g) > 2 else 0, float(g[3]) if len(g) > 3 else 0.0)
     return arr
 
 
@@ -246,6 +273,18 @@ class Engine:
         _check(lib().svmi_adjoint(self._h, _ptr(t), len(t), _ptr(ps), int(n_params), ctypes.byref(e),
                                   _ptr(grad) if n_params else None))
         return e.value, grad
+
+    def qir_run(self, n_qubits, gates):
+        """QIR gates on the first 2^n_qubits amplitudes (svmi_qir_run).
+        Returns (probs, outcomes, last_prob): P(1) and outcome per Measure."""
+        g = as_qir_gates(gates)
+        nm = int(np.count_nonzero(g["op"] == QIR_OP["Measure"]))
+        probs = np.zeros(max(nm, 1))
+        outs = np.zeros(max(nm, 1), dtype=np.int32)
+        last = ctypes.c_double(0.0)
+        _check(lib().svmi_qir_run(self._h, int(n_qubits), _ptr(g), len(g), _ptr(probs), _ptr(outs),
+                                  ctypes.byref(last)))
+        return probs[:nm], outs[:nm], last.value
 
     def set_option(self, option, value):
         _check(lib().svmi_set_option(self._h, int(option), int(value)))
@@ -370,3 +409,28 @@ def plan(n, gates, fusion=True, tile_qubits=12, low_qubits=4, sweep_budget=0):
     _check(lib().svmi_debug_plan(int(n), _ptr(g), len(g), int(fusion), int(tile_qubits), int(low_qubits),
                                  int(sweep_budget), ctypes.byref(passes), buf, len(buf)))
     return passes.value, buf.value.decode()
+
+
+def emulate_qir(n, gates, state, tile_qubits=12, low_qubits=4):
+    """Host execution of svmi_qir_run's program (planner check).
+    Returns (re, im, probs, outcomes)."""
+    g = as_qir_gates(gates)
+    re = np.array(state[0], dtype=np.float64, copy=True)
+    im = np.array(state[1], dtype=np.float64, copy=True)
+    nm = int(np.count_nonzero(g["op"] == QIR_OP["Measure"]))
+    probs = np.zeros(max(nm, 1))
+    outs = np.zeros(max(nm, 1), dtype=np.int32)
+    _check(lib().svmi_debug_emulate_qir(int(n), _ptr(g), len(g), int(tile_qubits), int(low_qubits), _ptr(re),
+                                        _ptr(im), _ptr(probs), _ptr(outs)))
+    return re, im, probs[:nm], outs[:nm]
+
+
+def plan_qir(n, gates, tile_qubits=12, low_qubits=4):
+    """(sweeps, measurements, description) of svmi_qir_run's program."""
+    g = as_qir_gates(gates)
+    sw = ctypes.c_uint64(0)
+    nm = ctypes.c_uint64(0)
+    buf = ctypes.create_string_buffer(1 << 20)
+    _check(lib().svmi_debug_plan_qir(int(n), _ptr(g), len(g), int(tile_qubits), int(low_qubits), ctypes.byref(sw),
+                                     ctypes.byref(nm), buf, len(buf)))
+    return sw.value, nm.value, buf.value.decode()

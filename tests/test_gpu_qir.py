@@ -1,0 +1,170 @@
+"""GPU parity of the QIR gate set: multi-controlled gates and mid-circuit
+Pauli measurement with collapse (svmi_qir_run) against the oracle
+(oracle/qir_oracle.c, pinned bit-exact to svsim_qsharp/svsim_cpu_omp.hpp)
+and the reference's fixtures.
+
+Bar: amplitudes and P(1) within 1e-10 max abs error of the FP64 oracle;
+outcomes equal (the programs' draws are chosen at least 1e-9 away from
+every P(1), where the two reductions' rounding cannot flip the comparison).
+At sizes the oracle cannot reach in seconds the checks are properties: the
+norm after every collapse, a second Z measurement of a measured qubit
+(P(1) exactly 0 or 1), and measurement of a GHZ state.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import svmi
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+sys.path.insert(0, GOLD)
+import make_qir_golden as MG  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+Q = svmi.QIR_OP
+
+
+def _err(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+def _safe_program(n, ng, nm, seed):
+    while True:
+        rng = np.random.default_rng(seed)
+        g = MG.random_program(n, ng, nm, rng)
+        re, im = MG.random_state(n, rng)
+        r, i = re.copy(), im.copy()
+        p, _, _ = O.qir_apply(n, r, i, g, nthreads=8)
+        if np.all(np.abs(g["theta"][g["op"] == Q["Measure"]] - p) > 1e-9):
+            return g, (re, im), (r, i, p)
+        seed += 1000
+
+
+def test_golden_program_n8():
+    d = np.load(os.path.join(GOLD, "qir_program_n8.npz"))
+    print("case: qir_program_n8", flush=True)
+    with svmi.Engine(8) as e:
+        p, o, last = e.qir_run(8, d["gates"])
+        re, im = e.get_state()
+    assert _err(re, d["re"]) < TOL and _err(im, d["im"]) < TOL
+    assert _err(p, d["prob"]) < TOL and np.array_equal(o, d["outcome"])
+    assert abs(last - d["prob"][-1]) < TOL
+
+
+def test_golden_kats_every_op():
+    d = np.load(os.path.join(GOLD, "qir_kats.npz"))
+    cap = int(d["capacity"])
+    print("case: qir_kats", flush=True)
+    with svmi.Engine(cap) as e:
+        for i, g in enumerate(d["gates"]):
+            e.set_state(d["in_re"][i], d["in_im"][i])
+            p, o, _ = e.qir_run(int(d["n"][i]), np.array([g], dtype=svmi.QIR_GATE_DTYPE))
+            re, im = e.get_state()
+            assert _err(re, d["out_re"][i]) < TOL and _err(im, d["out_im"][i]) < TOL, (O.QIR_OPS[g["op"]], i)
+            if g["op"] == Q["Measure"]:
+                assert abs(p[0] - d["prob"][i]) < TOL and o[0] == d["outcome"][i]
+
+
+@pytest.mark.parametrize("n,tile", [(3, 12), (6, 4), (10, 6), (14, 12), (16, 8), (18, 12)])
+def test_random_programs_vs_oracle(n, tile):
+    g, st, (r, i, p) = _safe_program(n, 400, 16, seed=500 + n)
+    print(f"case: random QIR program n={n} tile={tile}", flush=True)
+    with svmi.Engine(n) as e:
+        e.set_option(svmi.OPT_TILE_QUBITS, tile)
+        e.set_state(*st)
+        gp, go, last = e.qir_run(n, g)
+        re, im = e.get_state()
+    assert _err(re, r) < TOL and _err(im, i) < TOL
+    assert _err(gp, p) < TOL and abs(last - p[-1]) < TOL
+    assert np.array_equal(go, (g["theta"][g["op"] == Q["Measure"]] < p).astype(np.int32))
+
+
+def test_prefix_of_a_larger_state_and_last_prob():
+    # 9 allocated qubits inside a 12-qubit handle: only the first 2^9
+    # amplitudes move; last_prob persists across calls until reset
+    n, cap = 9, 12
+    g, st, (r, i, p) = _safe_program(n, 200, 6, seed=77)
+    rng = np.random.default_rng(1)
+    fre = rng.standard_normal(1 << cap) * 0.01
+    fim = rng.standard_normal(1 << cap) * 0.01
+    fre[: 1 << n], fim[: 1 << n] = st
+    print("case: prefix n=9 in 12-qubit handle", flush=True)
+    with svmi.Engine(cap) as e:
+        e.set_state(fre, fim)
+        gp, go, last = e.qir_run(n, g)
+        re, im = e.get_state()
+        assert _err(re[: 1 << n], r) < TOL and _err(im[: 1 << n], i) < TOL
+        assert np.array_equal(re[1 << n:], fre[1 << n:]) and np.array_equal(im[1 << n:], fim[1 << n:])
+        _, _, last2 = e.qir_run(n, [("H", 0)])          # no Measure: unchanged
+        assert last2 == last
+        e.reset_state()
+        _, _, last3 = e.qir_run(n, [])
+        assert last3 == 0.0
+
+
+def test_few_qubits_in_a_larger_handle():
+    # n < 4 allocated qubits run inside a 4-qubit tile; the rest stays zero
+    for n in (1, 2, 3):
+        rng = np.random.default_rng(n)
+        g = [("H", 0), ("RY", n - 1, 0, 0.3), ("Measure", 0, 1, 0.4), ("EX", 0, 0, 0.2)]
+        if n > 1:
+            g += [("ControlledX", n - 1, 1, 0.0), ("Swap", 0, n - 1, 0.0), ("Measure", n - 1, 2, 0.7)]
+        st = MG.random_state(n, rng)
+        r, i = st[0].copy(), st[1].copy()
+        p, o, _ = O.qir_apply(n, r, i, g, nthreads=1)
+        assert np.all(np.abs(np.array([x[3] for x in g if x[0] == "Measure"]) - p) > 1e-9)
+        print(f"case: {n}-qubit program in 6-qubit handle", flush=True)
+        with svmi.Engine(6) as e:
+            full_re = np.zeros(64)
+            full_im = np.zeros(64)
+            full_re[: 1 << n], full_im[: 1 << n] = st
+            e.set_state(full_re, full_im)
+            gp, go, _ = e.qir_run(n, g)
+            re, im = e.get_state()
+        assert _err(re[: 1 << n], r) < TOL and _err(im[: 1 << n], i) < TOL
+        assert np.all(re[1 << n:] == 0) and np.all(im[1 << n:] == 0)
+        assert _err(gp, p) < TOL and np.array_equal(go, o)
+
+
+def test_errors_leave_state_alone():
+    with svmi.Engine(5) as e:
+        e.set_random_state(3)
+        before = e.get_state()
+        for g in ([(99, 0, 0, 0.0)], [("X", 5, 0, 0.0)], [("Swap", 2, 2, 0.0)]):
+            with pytest.raises(svmi.SvmiError):
+                e.qir_run(5, g)
+        with pytest.raises(svmi.SvmiError):
+            e.qir_run(6, [])
+        after = e.get_state()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+
+
+def test_large_state_properties():
+    # 26 qubits: GHZ preparation with multi-controlled gates, then mid-circuit
+    # measurements; checked through closed forms
+    n = 26
+    g = [("H", 0)] + [("ControlledX", q, 1 << (q - 1), 0.0) for q in range(1, n)]
+    g += [("Measure", 13, 0, 0.3)]                       # GHZ: P(1) = 1/2, outcome 1
+    g += [("Measure", 25, 0, 0.5), ("Measure", 0, 1, 0.2)]
+    g += [("ControlledX", 0, ((1 << n) - 1) & ~1, 0.0)]  # 25 controls
+    print("case: 26-qubit GHZ with mid-circuit measurements", flush=True)
+    with svmi.Engine(n) as e:
+        p, o, last = e.qir_run(n, g)
+        tot = e.measure(np.array([]))[1]
+        amps = e.get_amplitudes([0, (1 << n) - 1, (1 << n) - 2])
+    assert abs(p[0] - 0.5) < 1e-12 and o[0] == 1
+    assert abs(p[1] - 1.0) < 1e-12 and o[1] == 1       # GHZ collapsed: all ones
+    assert abs(p[2] - 0.5) < 1e-12                      # X basis of a Z eigenstate
+    assert abs(tot - 1.0) < 1e-12
+    # after the X-basis outcome the state is |1..1> (x) |+/-> on qubit 0;
+    # the 25-control X on qubit 0 maps |-> -> -|->, |+> -> |+>
+    s = 1 / np.sqrt(2)
+    a_last = amps[0][1] + 1j * amps[1][1]
+    a_prev = amps[0][2] + 1j * amps[1][2]
+    assert abs(abs(a_last) - s) < 1e-12 and abs(abs(a_prev) - s) < 1e-12
+    assert abs(amps[0][0]) < 1e-12
