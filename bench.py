@@ -12,8 +12,10 @@ the CPU oracle (oracle/, OpenMP) on a bounded prefix of the same circuit.
 
 The "extra" object reports the other workloads: the 33-qubit QFT circuit
 (gates/s, sweep GB/s), the adjoint gradient of the 20-qubit 320-parameter
-ansatz (time vs one forward pass and vs GPU parameter shift) and 512 batched
-ansatz gradients vs a per-circuit loop.
+ansatz (time vs one forward pass and vs GPU parameter shift), 512 batched
+ansatz gradients vs a per-circuit loop, and the 30-qubit QIR gate-set
+program with multi-controlled gates and 16 mid-circuit measurements
+(gates/s, sweep and measurement GB/s, and the CPU oracle on a prefix of it).
 """
 import argparse
 import json
@@ -157,6 +159,64 @@ def bench_batch(count=512):
             "speedup": t_loop / t_batch, "max_abs_diff": max(diff, float(np.abs(ev - ev2).max()))}
 
 
+def bench_qir(n=30, reps=3):
+    """QIR gate-set program (svmi_qir_run): 1200 records incl. 1-3-control
+    and (n-1)-control gates and 16 mid-circuit measurements, from |0..0>.
+    Wall time per program (planning, upload, device work, one sync) and,
+    with HIP events per step, the average sweep and measurement durations."""
+    import svmi
+    from svmi import workloads as W
+    g = W.qir_mixed(n=n)
+    nm = int(np.count_nonzero(g["op"] == svmi.QIR_OP["Measure"]))
+    e = svmi.Engine(n)
+    e.qir_run(n, g)
+    walls, sweep_ms, sweep_n, meas_ms, meas_n, dev = [], 0.0, 0, 0.0, 0, []
+    for prof in (0, 1):
+        e.set_option(svmi.OPT_PROFILE, prof)
+        for _ in range(reps):
+            e.reset_state()
+            t0 = time.perf_counter()
+            _, outs, _ = e.qir_run(n, g)
+            t = time.perf_counter() - t0
+            st = e.stats()
+            if prof:
+                sweep_ms += st["sweep_ms"]
+                sweep_n += st["sweep_launches"]
+                meas_ms += st["other_ms"]
+                meas_n += st["other_launches"]
+            else:
+                walls.append(t)
+                dev.append(st["device_ms"])
+    e.close()
+    w = float(np.median(walls))
+    avg_sweep = sweep_ms / max(sweep_n, 1) / 1e3
+    avg_meas = meas_ms / max(meas_n, 1) / 1e3
+    return {"qubits": n, "records": len(g), "measures": nm, "ms_per_program": w * 1e3,
+            "device_ms": float(np.median(dev)), "gates_per_s": len(g) / w, "sweeps": sweep_n // reps,
+            "sweep_ms": avg_sweep * 1e3, "sweep_GBps": 32.0 * (1 << n) / avg_sweep / 1e9,
+            "measure_ms": avg_meas * 1e3, "measure_GBps": 16.0 * (1 << (n - 1)) / avg_meas / 1e9,
+            "ones": int(outs.sum())}
+
+
+def cpu_baseline_qir(n, budget_s=10.0, threads=None):
+    """Oracle restatement of the reference QIR engine (bit-identical to it,
+    OpenMP) on the first records of the same program, until ~budget_s."""
+    from oracle import oracle as O
+    from svmi import workloads as W
+    threads = threads or min(16, os.cpu_count() or 1)
+    g = W.qir_mixed(n=n)
+    re, im = O.zero_state(n)
+    done, t = 0, 0.0
+    while t < budget_s and done < len(g):
+        chunk = g[done:done + 4]
+        t0 = time.perf_counter()
+        O.qir_apply(n, re, im, chunk, nthreads=threads)
+        t += time.perf_counter() - t0
+        done += len(chunk)
+    return {"value": done / t, "unit": "records/s", "cores": threads, "kind": "port",
+            "sample": f"first {done} of {len(g)} records of the {n}-qubit QIR program, {t:.1f} s"}
+
+
 def measured_traffic(n):
     """HBM bytes per sweep launch of the default circuit, from the committed
     FETCH_SIZE / WRITE_SIZE counter runs (scaled by a copy of known size);
@@ -234,7 +294,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
-    ap.add_argument("--only", choices=("qft", "adjoint", "batch"),
+    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir"),
                     help="run only this extra workload and print its result (profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (HBM counter calibration)")
@@ -248,7 +308,8 @@ def main():
     if a.copy_gib > 0:
         calibration_copy(a.copy_gib)
     if a.only:
-        fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch}[a.only]
+        fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch,
+              "qir": lambda: bench_qir(30, reps=a.steps)}[a.only]
         log(f"[bench] only {a.only}")
         print(json.dumps({a.only: fn()}), flush=True)
         return
@@ -274,6 +335,12 @@ def main():
         log("[bench] 512 batched ansatz gradients vs per-circuit loop")
         extra["batch_ising_512"] = bench_batch()
         log(f"[bench] {extra['batch_ising_512']}")
+        log("[bench] QIR gate-set program, 30 qubits, 16 mid-circuit measurements")
+        extra["qir_mixed_n30"] = bench_qir(30)
+        log(f"[bench] {extra['qir_mixed_n30']}")
+        if not a.no_cpu:
+            extra["qir_mixed_n30"]["cpu_baseline"] = cpu_baseline_qir(30)
+            log(f"[bench] qir cpu {extra['qir_mixed_n30']['cpu_baseline']}")
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, OpenMP)")

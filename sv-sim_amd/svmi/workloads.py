@@ -13,14 +13,18 @@ gives the same circuit on every machine.
   hea_ising       hardware-efficient ansatz RY+RZ layers + CX ladder, Ising
                   observable sum J_i Z_i Z_{i+1} + sum h_i X_i (20 q, 8 layers)
   batch_ising     512 ansatz circuits, n ~ U{10..22}, layers ~ U{2..10}
+  qir_mixed       QIR gate-set program (svsim_qsharp engine records): single-
+                  qubit gates, 1-3-control gates, (n-1)-control X/Z, Swap and
+                  mid-circuit measurements in the Z, X and Y bases (30 q)
 """
 import numpy as np
 
-from . import GATE_DTYPE, OP, PAULI_DTYPE
+from . import GATE_DTYPE, OP, PAULI_DTYPE, QIR_GATE_DTYPE, QIR_OP
 
 RAND_MAX = 2147483647  # glibc; the reference draws rand()/RAND_MAX
 
-SEEDS = {"random_layers": 1001, "u3_brickwall": 1002, "qft": 1003, "hea_ising": 1004, "batch_ising": 1005}
+SEEDS = {"random_layers": 1001, "u3_brickwall": 1002, "qft": 1003, "hea_ising": 1004, "batch_ising": 1005,
+         "qir_mixed": 1006}
 
 
 class Circuit:
@@ -161,3 +165,42 @@ def draws(shots, seed):
     rng = np.random.default_rng(seed)
     k = rng.integers(0, RAND_MAX, size=shots, endpoint=True).astype(np.int64)
     return k, k.astype(np.float64) / float(RAND_MAX)
+
+
+def qir_mixed(n=30, n_gates=1200, n_measures=16, seed=SEEDS["qir_mixed"]):
+    """QIR gate-set program (QIR_GATE_DTYPE records): 60% uncontrolled
+    single-qubit gates of every kind, 30% gates with 1-3 random controls, 5%
+    X or Z with all n-1 other qubits as controls (Grover-style), 5% Swap; a
+    measurement every n_gates/n_measures records, bases cycling Z, X, Y, with
+    draws k/RAND_MAX."""
+    rng = np.random.default_rng(seed)
+    single = [QIR_OP[x] for x in ("X", "Y", "Z", "H", "S", "T", "RI", "RX", "RY", "RZ", "EI", "EX", "EY", "EZ",
+                                  "AdjointS", "AdjointT")]
+    ctl = [QIR_OP["Controlled" + x] for x in ("X", "Y", "Z", "H", "S", "T", "RI", "RX", "RY", "RZ", "EI", "EX",
+                                              "EY", "EZ", "AdjointS", "AdjointT")]
+    every = max(1, n_gates // max(n_measures, 1))
+    rows = []
+    m = 0
+    for i in range(n_gates):
+        if n_measures and (i + 1) % every == 0 and m < n_measures:
+            k = int(rng.integers(0, RAND_MAX, endpoint=True))
+            rows.append((QIR_OP["Measure"], int(rng.integers(n)), m % 3, k / float(RAND_MAX)))
+            m += 1
+            continue
+        u = rng.random()
+        q = int(rng.integers(n))
+        theta = float(rng.uniform(-2 * np.pi, 2 * np.pi))
+        if u < 0.60:
+            rows.append((int(rng.choice(single)), q, 0, theta))
+        elif u < 0.90:
+            others = np.delete(np.arange(n), q)
+            mask = 0
+            for c in rng.choice(others, size=int(rng.integers(1, 4)), replace=False):
+                mask |= 1 << int(c)
+            rows.append((int(rng.choice(ctl)), q, mask, theta))
+        elif u < 0.95:
+            op = QIR_OP["ControlledX"] if rng.random() < 0.5 else QIR_OP["ControlledZ"]
+            rows.append((op, q, ((1 << n) - 1) & ~(1 << q), 0.0))
+        else:
+            rows.append((QIR_OP["Swap"], q, int(rng.choice(np.delete(np.arange(n), q))), 0.0))
+    return np.array(rows, dtype=QIR_GATE_DTYPE)

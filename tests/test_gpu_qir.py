@@ -168,3 +168,20 @@ def test_large_state_properties():
     a_prev = amps[0][2] + 1j * amps[1][2]
     assert abs(abs(a_last) - s) < 1e-12 and abs(abs(a_prev) - s) < 1e-12
     assert abs(amps[0][0]) < 1e-12
+
+
+def test_qir_mixed_workload_n20():
+    # the benchmark's program shape at 20 qubits: (n-1)-control X/Z, 1-3
+    # control gates, Swap and 16 measurements, from |0..0>
+    from svmi import workloads as W
+    n = 20
+    g = W.qir_mixed(n=n)
+    re, im = O.zero_state(n)
+    p, o, _ = O.qir_apply(n, re, im, g, nthreads=8)
+    assert np.min(np.abs(g["theta"][g["op"] == Q["Measure"]] - p)) > 1e-9
+    print("case: qir_mixed n=20", flush=True)
+    with svmi.Engine(n) as e:
+        gp, go, _ = e.qir_run(n, g)
+        gre, gim = e.get_state()
+    assert _err(gre, re) < TOL and _err(gim, im) < TOL
+    assert _err(gp, p) < TOL and np.array_equal(go, o)
