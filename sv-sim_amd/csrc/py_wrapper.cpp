@@ -3,12 +3,20 @@
 // n_dev), append/upload/clear_circuit/run/reset/measure and the 38 static
 // gate factories.  Additions (no reference counterpart): the OP enum,
 // get_sv/set_sv, measure_with(draws), expval, adjoint, last_sim_ms.
+//
+// QirSimulator: the QIR runtime driver (svsim_qir.hpp, the reference's
+// SVSimSimulator in svsim_qsharp/svsim_qsharp_wrapper.cpp) with the same
+// method names; qubits and results are integer handles, qubit and Pauli
+// arrays are Python sequences.  A QirBackend subclass written in Python can
+// stand in for the GPU engine.
+#include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
 #include "../../include/svmi.h"
 #include "svsim_amd.hpp"
+#include "svsim_qir.hpp"
 
 namespace py = pybind11;
 using namespace SVSim;
@@ -23,6 +31,40 @@ std::vector<svmi_pauli_term> to_terms(const py::iterable& terms)
         if (tup.size() != 3) throw std::runtime_error("Pauli term must be (coeff, xmask, zmask)");
         out.push_back(svmi_pauli_term{tup[0].cast<double>(), tup[1].cast<uint64_t>(), tup[2].cast<uint64_t>()});
     }
+    return out;
+}
+
+namespace qir = SVSim::qir;
+
+// Python subclasses of QirBackend
+struct PyQirBackend : qir::Backend {
+    qir::RunResult run(int n, const std::vector<svmi_qir_gate>& gates) override
+    {
+        PYBIND11_OVERRIDE_PURE(qir::RunResult, qir::Backend, run, n, gates);
+    }
+    void reset() override { PYBIND11_OVERRIDE_PURE(void, qir::Backend, reset, ); }
+    int capacity() const override { PYBIND11_OVERRIDE_PURE(int, qir::Backend, capacity, ); }
+};
+
+// Owned by the simulator; keeps the Python backend object alive.
+struct HeldBackend : qir::Backend {
+    py::object keep;
+    qir::Backend* b;
+    explicit HeldBackend(py::object o) : keep(std::move(o)), b(keep.cast<qir::Backend*>()) {}
+    qir::RunResult run(int n, const std::vector<svmi_qir_gate>& gates) override { return b->run(n, gates); }
+    void reset() override { b->reset(); }
+    int capacity() const override { return b->capacity(); }
+};
+
+qir::Qubit to_q(uint64_t v) { return reinterpret_cast<qir::Qubit>(uintptr_t(v)); }
+uint64_t from_q(qir::Qubit q) { return uint64_t(reinterpret_cast<uintptr_t>(q)); }
+qir::Result to_r(uint64_t v) { return reinterpret_cast<qir::Result>(uintptr_t(v)); }
+uint64_t from_r(qir::Result r) { return uint64_t(reinterpret_cast<uintptr_t>(r)); }
+
+std::vector<qir::Qubit> to_qs(const std::vector<uint64_t>& v)
+{
+    std::vector<qir::Qubit> out;
+    for (uint64_t x : v) out.push_back(to_q(x));
     return out;
 }
 
@@ -120,4 +162,107 @@ PYBIND11_MODULE(libsvsim, m)
         .def_static("SRN", &Simulation::SRN)
         .def_static("W", &Simulation::W)
         .def_static("RYY", &Simulation::RYY);
+
+    // ---------------- QIR runtime driver ----------------
+    py::enum_<qir::PauliId>(m, "PauliId")
+        .value("PauliId_I", qir::PauliId_I)
+        .value("PauliId_X", qir::PauliId_X)
+        .value("PauliId_Z", qir::PauliId_Z)
+        .value("PauliId_Y", qir::PauliId_Y);
+    py::enum_<qir::ResultValue>(m, "ResultValue")
+        .value("Result_Zero", qir::Result_Zero)
+        .value("Result_One", qir::Result_One)
+        .value("Result_Pending", qir::Result_Pending);
+
+    py::class_<svmi_qir_gate>(m, "QirGate")
+        .def(py::init([](int32_t op, int32_t qubit, uint64_t mask, double theta) {
+                 return svmi_qir_gate{op, qubit, mask, theta};
+             }),
+             py::arg("op"), py::arg("qubit"), py::arg("mask") = 0, py::arg("theta") = 0.0)
+        .def_readwrite("op", &svmi_qir_gate::op)
+        .def_readwrite("qubit", &svmi_qir_gate::qubit)
+        .def_readwrite("mask", &svmi_qir_gate::mask)
+        .def_readwrite("theta", &svmi_qir_gate::theta)
+        .def("__repr__", [](const svmi_qir_gate& g) {
+            return "QirGate(" + std::to_string(g.op) + ", " + std::to_string(g.qubit) + ", " +
+                   std::to_string(g.mask) + ", " + std::to_string(g.theta) + ")";
+        });
+
+    py::class_<qir::Backend, PyQirBackend>(m, "QirBackend")
+        .def(py::init<>())
+        .def("run", &qir::Backend::run)
+        .def("reset", &qir::Backend::reset)
+        .def("capacity", &qir::Backend::capacity);
+
+    using QS = qir::SVSimSimulator;
+    auto ctl = [](void (QS::*f)(long, qir::Qubit[], qir::Qubit)) {
+        return [f](QS& s, const std::vector<uint64_t>& c, uint64_t t) {
+            std::vector<qir::Qubit> cs = to_qs(c);
+            (s.*f)(long(cs.size()), cs.data(), to_q(t));
+        };
+    };
+    auto one = [](void (QS::*f)(qir::Qubit)) { return [f](QS& s, uint64_t t) { (s.*f)(to_q(t)); }; };
+    py::class_<QS>(m, "QirSimulator")
+        .def(py::init<int, int>(), py::arg("capacity") = 28, py::arg("device") = 0)
+        .def(py::init([](py::object backend) { return new QS(std::unique_ptr<qir::Backend>(new HeldBackend(backend))); }),
+             py::arg("backend"))
+        .def("set_draw", &QS::set_draw)
+        .def("AllocateQubit", [](QS& s) { return from_q(s.AllocateQubit()); })
+        .def("ReleaseQubit", [](QS& s, uint64_t q) { s.ReleaseQubit(to_q(q)); })
+        .def("QubitToString", [](QS& s, uint64_t q) { return s.QubitToString(to_q(q)); })
+        .def("ReleaseResult", [](QS& s, uint64_t r) { s.ReleaseResult(to_r(r)); })
+        .def("AreEqualResults", [](QS& s, uint64_t a, uint64_t b) { return s.AreEqualResults(to_r(a), to_r(b)); })
+        .def("GetResultValue", [](QS& s, uint64_t r) { return s.GetResultValue(to_r(r)); })
+        .def("UseZero", [](QS& s) { return from_r(s.UseZero()); })
+        .def("UseOne", [](QS& s) { return from_r(s.UseOne()); })
+        .def("X", one(&QS::X))
+        .def("Y", one(&QS::Y))
+        .def("Z", one(&QS::Z))
+        .def("H", one(&QS::H))
+        .def("S", one(&QS::S))
+        .def("T", one(&QS::T))
+        .def("AdjointS", one(&QS::AdjointS))
+        .def("AdjointT", one(&QS::AdjointT))
+        .def("R", [](QS& s, qir::PauliId axis, uint64_t t, double theta) { s.R(axis, to_q(t), theta); })
+        .def("Exp", [](QS& s, std::vector<qir::PauliId> paulis, const std::vector<uint64_t>& targets, double theta) {
+            if (paulis.size() != targets.size()) throw std::runtime_error("Exp: paulis and targets differ in length");
+            std::vector<qir::Qubit> ts = to_qs(targets);
+            s.Exp(long(ts.size()), paulis.data(), ts.data(), theta);
+        })
+        .def("ControlledX", ctl(&QS::ControlledX))
+        .def("ControlledY", ctl(&QS::ControlledY))
+        .def("ControlledZ", ctl(&QS::ControlledZ))
+        .def("ControlledH", ctl(&QS::ControlledH))
+        .def("ControlledS", ctl(&QS::ControlledS))
+        .def("ControlledT", ctl(&QS::ControlledT))
+        .def("ControlledAdjointS", ctl(&QS::ControlledAdjointS))
+        .def("ControlledAdjointT", ctl(&QS::ControlledAdjointT))
+        .def("ControlledR", [](QS& s, const std::vector<uint64_t>& c, qir::PauliId axis, uint64_t t, double theta) {
+            std::vector<qir::Qubit> cs = to_qs(c);
+            s.ControlledR(long(cs.size()), cs.data(), axis, to_q(t), theta);
+        })
+        .def("ControlledExp", [](QS& s, const std::vector<uint64_t>& c, std::vector<qir::PauliId> paulis,
+                                 const std::vector<uint64_t>& targets, double theta) {
+            if (paulis.size() != targets.size())
+                throw std::runtime_error("ControlledExp: paulis and targets differ in length");
+            std::vector<qir::Qubit> cs = to_qs(c), ts = to_qs(targets);
+            s.ControlledExp(long(cs.size()), cs.data(), long(ts.size()), paulis.data(), ts.data(), theta);
+        })
+        .def("Measure", [](QS& s, std::vector<qir::PauliId> bases, const std::vector<uint64_t>& targets) {
+            std::vector<qir::Qubit> ts = to_qs(targets);
+            return from_r(s.Measure(long(bases.size()), bases.data(), long(ts.size()), ts.data()));
+        })
+        .def("num_qubits", &QS::num_qubits)
+        .def("slot_of", [](const QS& s, uint64_t q) { return s.slot_of(to_q(q)); })
+        .def("last_prob", &QS::last_prob)
+        .def("pending", &QS::pending)
+        .def("flush", &QS::flush)
+        .def("get_sv", [](QS& s) {
+            auto* d = dynamic_cast<qir::DeviceBackend*>(&s.backend());
+            if (!d) throw std::runtime_error("get_sv: the simulator does not run on the GPU engine");
+            const size_t dim = size_t(1) << d->capacity();
+            py::array_t<double> re(dim), im(dim);
+            svmi_throw(svmi_get_state(d->handle(), re.mutable_data(), im.mutable_data()), "svmi_get_state");
+            return py::make_tuple(re, im);
+        });
 }

@@ -185,3 +185,46 @@ def test_qir_mixed_workload_n20():
         gre, gim = e.get_state()
     assert _err(gre, re) < TOL and _err(gim, im) < TOL
     assert _err(gp, p) < TOL and np.array_equal(go, o)
+
+
+def test_runtime_driver_on_gpu_matches_oracle_engine():
+    # the same QIR-runtime call sequence through libsvsim.QirSimulator on the
+    # GPU engine and on the oracle (tests/test_qir_driver.py's backend)
+    import libsvsim
+    from test_qir_driver import OracleBackend
+    P = libsvsim.PauliId
+    cap = 10
+    draws = [0.37, 0.81, 0.05, 0.66, 0.5, 0.12, 0.93]
+
+    def program(sim):
+        it = iter(draws)
+        sim.set_draw(lambda: next(it))
+        qs = [sim.AllocateQubit() for _ in range(8)]
+        res = []
+        for q in qs:
+            sim.H(q)
+        sim.ControlledX(qs[:5], qs[7])
+        sim.Exp([P.PauliId_X, P.PauliId_Y, P.PauliId_Z], [qs[1], qs[3], qs[6]], 0.42)
+        sim.ControlledExp([qs[0], qs[2]], [P.PauliId_Z, P.PauliId_X], [qs[4], qs[5]], -0.7)
+        res.append(sim.Measure([P.PauliId_Z], [qs[2]]))
+        sim.ControlledR([qs[1]], P.PauliId_Y, qs[3], 1.1)
+        sim.T(qs[4])
+        sim.AdjointS(qs[5])
+        res.append(sim.Measure([P.PauliId_X], [qs[3]]))
+        sim.ReleaseQubit(qs[2])          # measured: swap into the last slot first
+        extra = sim.AllocateQubit()
+        sim.ControlledAdjointT([qs[0], extra], qs[6])
+        res.append(sim.Measure([P.PauliId_Y], [qs[6]]))
+        res.append(sim.Measure([P.PauliId_Z], [qs[0]]))
+        sim.flush()
+        return [int(sim.GetResultValue(r)) for r in res], sim.last_prob(), sim.num_qubits()
+
+    print("case: QIR runtime driver, GPU vs oracle engine", flush=True)
+    gpu = libsvsim.QirSimulator(cap, 0)
+    g_res = program(gpu)
+    g_re, g_im = gpu.get_sv()
+    be = OracleBackend(cap)
+    c_res = program(libsvsim.QirSimulator(backend=be))
+    assert g_res[0] == c_res[0] and g_res[2] == c_res[2]
+    assert abs(g_res[1] - c_res[1]) < TOL
+    assert _err(g_re, be.re) < TOL and _err(g_im, be.im) < TOL
