@@ -24,6 +24,10 @@ ARCH = os.environ.get("SVMI_ARCH", "gfx950")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # branches on uniform (scalar) conditions -- the sweep kernels' op
+    # dispatch -- stay unstructured: fewer register copies at the joins
+    # (u3 sweep at 30 qubits 17.1 -> 14.8 ms on MI355X)
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 ENGINE_SOURCES = ["svmi_api.hip", "svmi_kernels.hip", "svmi_lower.cpp", "svmi_plan.cpp", "svmi_qir.cpp"]
