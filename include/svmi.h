@@ -223,6 +223,26 @@ typedef struct svmi_qir_gate {
 svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates,
                          double* probs, int32_t* outcomes, double* last_prob);
 
+/* ---- OpenQASM 2.0 front end (replaces the translator
+ *      svsim/QASMBench/svsim_qasm.py, which writes a Python script of
+ *      sim.append(sim.OP(...)) calls).  Host only: parses `text` (len bytes)
+ *      into the same gate records, quantum registers laid out in declaration
+ *      order.  measure / reset / barrier / if statements make no gates.
+ *      gates may be NULL to query the size; otherwise it must hold
+ *      info->n_gates records (SVMI_ERR_ARG if capacity is smaller).  Parse
+ *      errors return SVMI_ERR_ARG with "line L: ..." in svmi_last_error. */
+typedef struct svmi_qasm_info {
+    int32_t n_qubits;        /* sum of the qreg sizes                                */
+    int32_t reserved;
+    uint64_t n_gates;        /* gate records produced                                 */
+    uint64_t basic_gates;    /* gate count by the translator's table (svsim_qasm.py:30-74) */
+    uint64_t cx_gates;       /* CX count by the translator's table (svsim_qasm.py:78-88)   */
+    uint64_t skipped;        /* measure / reset / barrier / if statements             */
+} svmi_qasm_info;
+
+svmi_status svmi_qasm_parse(const char* text, size_t len, svmi_gate* gates, size_t capacity,
+                            svmi_qasm_info* info);
+
 /* ---- options, stats, errors */
 svmi_status svmi_set_option(svmi_handle h, int option, int64_t value);
 svmi_status svmi_get_stats(svmi_handle h, svmi_stats* out);

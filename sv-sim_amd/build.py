@@ -30,7 +30,8 @@ HIP_FLAGS = [
     "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-ENGINE_SOURCES = ["svmi_api.hip", "svmi_kernels.hip", "svmi_lower.cpp", "svmi_plan.cpp", "svmi_qir.cpp"]
+ENGINE_SOURCES = ["svmi_api.hip", "svmi_kernels.hip", "svmi_lower.cpp", "svmi_plan.cpp", "svmi_qir.cpp",
+                  "svmi_qasm.cpp"]
 LIB = os.path.join(HERE, "libsvmi.so")
 
 

@@ -86,6 +86,7 @@ PYBIND11_MODULE(libsvsim, m)
             self.cast<Simulation&>().upload();
             return self;
         })
+        .def("append_qasm", &Simulation::append_qasm)
         .def("clear_circuit", &Simulation::clear_circuit)
         .def("run", &Simulation::sim, py::call_guard<py::gil_scoped_release>())
         .def("reset", &Simulation::reset)

@@ -16,6 +16,7 @@
 #include "svmi_kernels.hpp"
 #include "svmi_lower.hpp"
 #include "svmi_plan.hpp"
+#include "svmi_qasm.hpp"
 #include "svmi_qir.hpp"
 
 using namespace svmi;
@@ -915,6 +916,29 @@ svmi_status svmi_run(svmi_handle h)
     float ms = 0.f;
     SVMI_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.device_ms = ms;
+    return SVMI_OK;
+}
+
+svmi_status svmi_qasm_parse(const char* text, size_t len, svmi_gate* gates, size_t capacity, svmi_qasm_info* info)
+{
+    if (!text && len) return fail(SVMI_ERR_ARG, "null text");
+    if (!info) return fail(SVMI_ERR_ARG, "null info");
+    QasmProgram prog;
+    std::string err;
+    const svmi_status st = parse_qasm(text, len, prog, err);
+    if (st != SVMI_OK) return fail(st, "qasm: " + err);
+    info->n_qubits = prog.n_qubits;
+    info->reserved = 0;
+    info->n_gates = prog.gates.size();
+    info->basic_gates = prog.basic_gates;
+    info->cx_gates = prog.cx_gates;
+    info->skipped = prog.skipped;
+    if (gates) {
+        if (capacity < prog.gates.size())
+            return fail(SVMI_ERR_ARG, "qasm: " + std::to_string(prog.gates.size()) + " gates do not fit in " +
+                                          std::to_string(capacity));
+        std::copy(prog.gates.begin(), prog.gates.end(), gates);
+    }
     return SVMI_OK;
 }
 

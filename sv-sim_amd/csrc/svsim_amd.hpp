@@ -111,6 +111,24 @@ public:
         circuit.push_back(*g);
         n_gates++;
     }
+    // OpenQASM 2.0 source -> appended gates (no reference counterpart: the
+    // reference translates .qasm files to Python scripts of append calls,
+    // svsim/QASMBench/svsim_qasm.py).  Returns the number of gates appended.
+    size_t append_qasm(const std::string& text)
+    {
+        svmi_qasm_info info;
+        svmi_throw(svmi_qasm_parse(text.data(), text.size(), nullptr, 0, &info), "append_qasm");
+        if (IdxType(info.n_qubits) > n_qubits)
+            throw std::runtime_error("append_qasm: the circuit needs " + std::to_string(info.n_qubits) + " qubits");
+        std::vector<svmi_gate> recs(info.n_gates);
+        svmi_throw(svmi_qasm_parse(text.data(), text.size(), recs.data(), recs.size(), &info), "append_qasm");
+        for (const svmi_gate& r : recs) {
+            const Gate g(OP(r.op), IdxType(r.qb[0]), IdxType(r.qb[1]), IdxType(r.qb[2]), IdxType(r.qb[3]),
+                         IdxType(r.qb[4]), r.theta, r.phi, r.lambda);
+            append(&g);
+        }
+        return recs.size();
+    }
     // svsim_cpu_omp.hpp:256-275
     Simulation* upload()
     {

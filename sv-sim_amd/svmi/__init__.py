@@ -49,7 +49,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
 EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state", "svmi_set_random_state",
            "svmi_set_state", "svmi_get_state", "svmi_get_amplitudes", "svmi_upload", "svmi_clear_circuit",
            "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
-           "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version", "svmi_qir_run",
+           "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version", "svmi_qir_run", "svmi_qasm_parse",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
            "svmi_debug_plan_batch", "svmi_debug_plan_adjoint", "svmi_debug_emulate_qir", "svmi_debug_plan_qir"]
 
