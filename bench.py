@@ -15,7 +15,8 @@ The "extra" object reports the other workloads: the 33-qubit QFT circuit
 ansatz (time vs one forward pass and vs GPU parameter shift), 512 batched
 ansatz gradients vs a per-circuit loop, and the 30-qubit QIR gate-set
 program with multi-controlled gates and 16 mid-circuit measurements
-(gates/s, sweep and measurement GB/s, and the CPU oracle on a prefix of it).
+(gates/s, sweep and measurement GB/s, and the CPU oracle on a prefix of it),
+and the largest QASMBench circuits through the OpenQASM front end.
 """
 import argparse
 import json
@@ -198,6 +199,56 @@ def bench_qir(n=30, reps=3):
             "ones": int(outs.sum())}
 
 
+QASM_CIRCUITS = ("qft_n20", "square_root_n18", "multiplier_n25", "swap_test_n25", "ising_n26", "wstate_n27")
+
+
+def bench_qasm(reps=3):
+    """The reference's largest QASMBench circuits (tests/golden/qasm/):
+    OpenQASM parse time on the host, then median device time per run."""
+    import svmi
+    from svmi import qasm
+    out = {}
+    for name in QASM_CIRCUITS:
+        with open(os.path.join(ROOT, "tests", "golden", "qasm", name + ".qasm"), "rb") as f:
+            text = f.read()
+        t0 = time.perf_counter()
+        n, gates, info = qasm.parse(text)
+        t_parse = time.perf_counter() - t0
+        e = svmi.Engine(n)
+        t0 = time.perf_counter()
+        e.upload(gates)
+        t_plan = time.perf_counter() - t0
+        runs = []
+        for _ in range(reps):
+            e.reset_state()
+            t0 = time.perf_counter()
+            e.run()
+            runs.append(time.perf_counter() - t0)
+        st = e.stats()
+        e.close()
+        t = float(np.median(runs))
+        out[name] = {"qubits": n, "gates": len(gates), "parse_ms": t_parse * 1e3, "plan_ms": t_plan * 1e3,
+                     "run_ms": t * 1e3, "gates_per_s": len(gates) / t, "sweeps": st["sweeps"]}
+    return out
+
+
+def cpu_reference_qasm(name="qft_n20", threads=None):
+    """The reference CPU engine itself (oracle/_ref, compiled from
+    svsim/src/svsim_cpu_omp.hpp) on one whole QASMBench circuit; None when it
+    was not built."""
+    from oracle import oracle as O
+    from svmi import qasm
+    if not O.ref_available():
+        return None
+    threads = threads or min(16, os.cpu_count() or 1)
+    n, gates, _ = qasm.load(os.path.join(ROOT, "tests", "golden", "qasm", name + ".qasm"))
+    t0 = time.perf_counter()
+    O.ref_run(n, gates, nthreads=threads)
+    t = time.perf_counter() - t0
+    return {"value": len(gates) / t, "unit": "gates/s", "cores": threads, "kind": "reference",
+            "sample": f"{name} ({n} qubits, {len(gates)} gates), whole circuit, {t:.1f} s"}
+
+
 def cpu_baseline_qir(n, budget_s=10.0, threads=None):
     """Oracle restatement of the reference QIR engine (bit-identical to it,
     OpenMP) on the first records of the same program, until ~budget_s."""
@@ -294,7 +345,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
-    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir"),
+    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm"),
                     help="run only this extra workload and print its result (profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (HBM counter calibration)")
@@ -309,7 +360,7 @@ def main():
         calibration_copy(a.copy_gib)
     if a.only:
         fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch,
-              "qir": lambda: bench_qir(30, reps=a.steps)}[a.only]
+              "qir": lambda: bench_qir(30, reps=a.steps), "qasm": lambda: bench_qasm(reps=a.steps)}[a.only]
         log(f"[bench] only {a.only}")
         print(json.dumps({a.only: fn()}), flush=True)
         return
@@ -341,6 +392,11 @@ def main():
         if not a.no_cpu:
             extra["qir_mixed_n30"]["cpu_baseline"] = cpu_baseline_qir(30)
             log(f"[bench] qir cpu {extra['qir_mixed_n30']['cpu_baseline']}")
+        log("[bench] QASMBench circuits (OpenQASM front end)")
+        extra["qasmbench"] = bench_qasm()
+        if not a.no_cpu:
+            extra["qasmbench"]["cpu_baseline"] = cpu_reference_qasm()
+        log(f"[bench] {extra['qasmbench']}")
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, OpenMP)")
