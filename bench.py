@@ -16,7 +16,9 @@ ansatz (time vs one forward pass and vs GPU parameter shift), 512 batched
 ansatz gradients vs a per-circuit loop, and the 30-qubit QIR gate-set
 program with multi-controlled gates and 16 mid-circuit measurements
 (gates/s, sweep and measurement GB/s, and the CPU oracle on a prefix of it),
-and the largest QASMBench circuits through the OpenQASM front end.
+the largest QASMBench circuits through the OpenQASM front end, and the u3
+and QFT circuits again with sweeps compiled into straight-line kernels at
+upload (SVMI_OPT_SPECIALIZE; compile time reported beside the run time).
 """
 import argparse
 import json
@@ -91,6 +93,44 @@ def bench_qft(n, steps, seed=2033):
     avg = sweep_ms / max(sweep_n, 1) / 1e3
     return {"qubits": n, "gates": len(gates), "ms_per_circuit": dt * 1e3, "gates_per_s": len(gates) / dt,
             "passes": passes, "sweep_GBps": 32.0 * (1 << n) / avg / 1e9}
+
+
+def bench_specialized(steps=2):
+    """The u3 brick-wall (30 qubits) and QFT (33 qubits) circuits with
+    SVMI_OPT_SPECIALIZE: sweeps compiled into straight-line kernels with
+    hipRTC at upload.  Reports the upload (compile) time separately from the
+    per-circuit time (parity with the interpreted sweeps: tests)."""
+    import svmi
+    from svmi import workloads as W
+    out = {}
+    for name, n, gates in (("u3_brickwall_n30", 30, W.u3_brickwall(n=30, layers=20)), ("qft_n33", 33, W.qft(n=33))):
+        e = svmi.Engine(n)
+        e.set_random_state(2033)
+        e.set_option(svmi.OPT_SPECIALIZE, 1)
+        log(f"[bench] specialised {name}: compiling")
+        t0 = time.perf_counter()
+        e.upload(gates)
+        t_up = time.perf_counter() - t0
+        st = e.stats()
+        e.run()
+        e.set_option(svmi.OPT_PROFILE, 1)
+        sweep_ms, sweep_n, walls = 0.0, 0, []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            e.run()
+            walls.append(time.perf_counter() - t0)
+            s2 = e.stats()
+            sweep_ms += s2["sweep_ms"]
+            sweep_n += s2["sweep_launches"]
+        e.set_option(svmi.OPT_PROFILE, 0)
+        e.close()
+        dt = float(np.median(walls))
+        avg = sweep_ms / max(sweep_n, 1) / 1e3
+        out[name] = {"qubits": n, "gates": len(gates), "upload_s": t_up, "compile_s": st["compile_ms"] / 1e3,
+                     "specialized_sweeps": st["specialized_sweeps"], "ms_per_circuit": dt * 1e3,
+                     "gates_per_s": len(gates) / dt, "sweep_GBps": 32.0 * (1 << n) / avg / 1e9}
+        log(f"[bench] {out[name]}")
+    return out
 
 
 def bench_adjoint(n=20, layers=8, reps=5):
@@ -345,7 +385,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
-    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm"),
+    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm", "specialized"),
                     help="run only this extra workload and print its result (profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (HBM counter calibration)")
@@ -360,7 +400,8 @@ def main():
         calibration_copy(a.copy_gib)
     if a.only:
         fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch,
-              "qir": lambda: bench_qir(30, reps=a.steps), "qasm": lambda: bench_qasm(reps=a.steps)}[a.only]
+              "qir": lambda: bench_qir(30, reps=a.steps), "qasm": lambda: bench_qasm(reps=a.steps),
+              "specialized": lambda: bench_specialized(steps=a.steps)}[a.only]
         log(f"[bench] only {a.only}")
         print(json.dumps({a.only: fn()}), flush=True)
         return
@@ -380,6 +421,7 @@ def main():
         log("[bench] qft n=33 on a seeded random input")
         extra["qft_n33"] = bench_qft(33, steps=2)
         log(f"[bench] {extra['qft_n33']}")
+        extra["specialized"] = bench_specialized()
         log("[bench] adjoint gradient, 20-qubit ansatz, 320 parameters")
         extra["adjoint_hea_n20"] = bench_adjoint()
         log(f"[bench] {extra['adjoint_hea_n20']}")

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SVMI_ABI_VERSION 1
+#define SVMI_ABI_VERSION 2
 
 typedef enum svmi_status {
     SVMI_OK = 0,
@@ -86,6 +86,12 @@ typedef struct svmi_stats {
     double sweep_ms;         /* summed duration of those launches             */
     uint64_t other_launches; /* standalone-op kernels                         */
     double other_ms;
+    /* with SVMI_OPT_SPECIALIZE: sweeps of the uploaded circuit that run as
+     * compiled straight-line kernels (0 if compilation failed: the sweeps are
+     * then interpreted, and svmi_last_error names the failure) and the hipRTC
+     * compile time of the last upload                                         */
+    uint64_t specialized_sweeps;
+    double compile_ms;
 } svmi_stats;
 
 /* Engine options (svmi_set_option). */
@@ -98,8 +104,11 @@ typedef enum svmi_option {
     SVMI_OPT_PIPELINE = 5,    /* 1 = persistent sweeps prefetching the next tile,
                                  2 = the same with one 512-VGPR workgroup per CU, 0 = off (default) */
     SVMI_OPT_OP_COST = 6,     /* fusion cost model: per-op cost in FMA per amplitude (default 8) */
-    SVMI_OPT_COEF_BUDGET = 7  /* split a forward sweep whose op coefficients exceed this many
+    SVMI_OPT_COEF_BUDGET = 7, /* split a forward sweep whose op coefficients exceed this many
                                  bytes into consecutive sweeps (default 20000, 0 = never)   */
+    SVMI_OPT_SPECIALIZE = 8   /* 1 = compile each uploaded circuit's sweeps into straight-line
+                                 kernels with hipRTC at upload (run-time compilation, seconds
+                                 per circuit); 0 = interpret the sweep programs (default)     */
 } svmi_option;
 
 /* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,

@@ -57,6 +57,13 @@ svmi_status svmi_debug_emulate_qir(int n, const svmi_qir_gate* gates, size_t n_g
 svmi_status svmi_debug_plan_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
                                 uint64_t* n_sweeps, uint64_t* n_measures, char* buf, size_t buflen);
 
+/* Source of the specialised sweep kernels (SVMI_OPT_SPECIALIZE) for a
+ * circuit (buf, truncated to buflen-1 bytes; may be NULL) and, if compile
+ * != 0, its hipRTC compilation for gfx950 (no device needed): time and
+ * code-object size. */
+svmi_status svmi_debug_jit(int n, const svmi_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
+                           int compile, double* compile_ms, uint64_t* code_bytes, char* buf, size_t buflen);
+
 /* Planner summary: number of full-state passes and a text description
  * (truncated to buflen-1 bytes; buf may be NULL). */
 svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates,

@@ -16,6 +16,7 @@
 #include "svmi_kernels.hpp"
 #include "svmi_lower.hpp"
 #include "svmi_plan.hpp"
+#include "svmi_jit.hpp"
 #include "svmi_qasm.hpp"
 #include "svmi_qir.hpp"
 
@@ -156,6 +157,16 @@ svmi_status plan_qir(const svmi_qir_gate* gates, size_t n_gates, int n, int n_pr
         return fail(SVMI_ERR_ARG, std::string("QIR planning: ") + e.what());
     }
     return SVMI_OK;
+}
+
+// gfx target of the current device, without feature suffixes
+std::string device_arch()
+{
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return "gfx950";
+    std::string a = p.gcnArchName;
+    return a.substr(0, a.find(':'));
 }
 
 svmi_status check_terms(const svmi_pauli_term* terms, size_t n_terms, int n)
@@ -583,6 +594,9 @@ struct svmi_sim {
     bool uploaded = false;
     Program prog;
     DeviceProgram dprog;
+    JitSweeps jit;             // specialised kernels of prog's sweeps (SVMI_OPT_SPECIALIZE)
+    bool specialize = false;
+    std::string jit_error;     // why the last specialisation failed (the sweeps then interpret)
     PlanOptions opt;
     bool profile = false;
     int pipeline = 0;
@@ -627,7 +641,8 @@ svmi_status use_device(svmi_handle h)
 
 svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, c64* s0, c64* s1,
                          double* grad_sums /* device, prog.n_grad */,
-                         MeasureResult* meas = nullptr /* device, prog.measures.size() */)
+                         MeasureResult* meas = nullptr /* device, prog.measures.size() */,
+                         const JitSweeps* jit = nullptr /* specialised kernels of prog's sweeps */)
 {
     const bool prof = h->profile;
     if (prof) {
@@ -647,9 +662,12 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
                 SVMI_TRY(h->scratch[5].ensure(sw.n_tiles * sw.n_grad * sizeof(double)));
                 partials = h->scratch[5].as<double>();
             }
-            SVMI_HIP(launch_sweep(h->stream, prog.n_states, s0, s1, sw, dp.phases.as<PhaseInfo>(),
-                                  dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials,
-                                  h->pipeline));
+            if (jit && st.index < jit->fn.size() && jit->fn[st.index])
+                SVMI_HIP(jit_launch(h->stream, *jit, st.index, sw, s0, dp.coef.as<c64>(), dp.hitab.as<uint64_t>()));
+            else
+                SVMI_HIP(launch_sweep(h->stream, prog.n_states, s0, s1, sw, dp.phases.as<PhaseInfo>(),
+                                      dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials,
+                                      h->pipeline));
             if (partials)
                 SVMI_HIP(launch_sum_rows(h->stream, partials, sw.n_grad, sw.n_tiles, grad_sums + sw.grad_base));
         } else if (st.kind == Step::MEASURE) {
@@ -766,6 +784,7 @@ svmi_status svmi_destroy(svmi_handle h)
     for (auto& b : h->scratch) b.release();
     for (hipEvent_t e : h->prof_events) (void)hipEventDestroy(e);
     h->dprog.release();
+    h->jit.release();
     h->adj_dprog.release();
     h->adj_gsum.release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -877,6 +896,26 @@ svmi_status svmi_get_amplitudes(svmi_handle h, const uint64_t* idx, size_t count
     return SVMI_OK;
 }
 
+// Device copy of h->prog and, with SVMI_OPT_SPECIALIZE, its compiled sweeps.
+svmi_status install_program(svmi_handle h)
+{
+    SVMI_TRY(h->dprog.upload(h->prog, h->stream));
+    h->jit.release();
+    h->jit_error.clear();
+    h->stats.specialized_sweeps = 0;
+    h->stats.compile_ms = 0.0;
+    if (!h->specialize) return SVMI_OK;
+    if (!jit_build(h->prog, device_arch().c_str(), h->jit, h->jit_error)) {
+        h->jit.release();
+        // the sweeps are interpreted; the reason stays readable
+        if (!h->jit_error.empty()) g_err = "specialisation: " + h->jit_error;
+        return SVMI_OK;
+    }
+    for (hipFunction_t f : h->jit.fn) h->stats.specialized_sweeps += f != nullptr;
+    h->stats.compile_ms = h->jit.compile_ms;
+    return SVMI_OK;
+}
+
 svmi_status svmi_upload(svmi_handle h, const svmi_gate* gates, size_t n_gates)
 {
     SVMI_TRY(use_device(h));
@@ -884,7 +923,7 @@ svmi_status svmi_upload(svmi_handle h, const svmi_gate* gates, size_t n_gates)
     h->gates.assign(gates, gates + n_gates);
     h->adj_valid = false;
     plan_forward(gates, n_gates, h->n_int, h->opt, h->prog);
-    SVMI_TRY(h->dprog.upload(h->prog, h->stream));
+    SVMI_TRY(install_program(h));
     h->uploaded = true;
     h->stats.gates = n_gates;
     h->stats.ops = h->prog.n_lowered;
@@ -910,7 +949,7 @@ svmi_status svmi_run(svmi_handle h)
     h->stats.sweep_launches = h->stats.other_launches = 0;
     h->stats.sweep_ms = h->stats.other_ms = 0.0;
     SVMI_HIP(hipEventRecord(h->ev0, h->stream));
-    SVMI_TRY(exec_program(h, h->prog, h->dprog, h->sv, nullptr, nullptr));
+    SVMI_TRY(exec_program(h, h->prog, h->dprog, h->sv, nullptr, nullptr, nullptr, &h->jit));
     SVMI_HIP(hipEventRecord(h->ev1, h->stream));
     SVMI_HIP(hipEventSynchronize(h->ev1));
     float ms = 0.f;
@@ -1232,13 +1271,14 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
         if (value < 0 || value > 64) return fail(SVMI_ERR_ARG, "pipeline must be in [0, 64]");
         h->pipeline = int(value);
         return SVMI_OK;
+    case SVMI_OPT_SPECIALIZE: h->specialize = value != 0; break;
     default: return fail(SVMI_ERR_ARG, "unknown option " + std::to_string(option));
     }
     h->adj_valid = false;
     if (h->uploaded) {
         plan_forward(h->gates.data(), h->gates.size(), h->n_int, h->opt, h->prog);
         SVMI_TRY(use_device(h));
-        SVMI_TRY(h->dprog.upload(h->prog, h->stream));
+        SVMI_TRY(install_program(h));
         h->stats.ops = h->prog.n_lowered;
     }
     return SVMI_OK;
@@ -1583,6 +1623,32 @@ svmi_status svmi_debug_plan_batch(const svmi_batch_item* items, size_t n_items, 
     if (n_launches) *n_launches = bp.launches.size();
     if (n_tasks) *n_tasks = bp.n_tasks;
     if (state_bytes) *state_bytes = bp.state_bytes;
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_jit(int n, const svmi_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
+                           int compile, double* compile_ms, uint64_t* code_bytes, char* buf, size_t buflen)
+{
+    if (n < 1 || n > kMaxQubits) return fail(SVMI_ERR_ARG, "bad n");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    SVMI_TRY(check_gates(gates, n_gates, n));
+    Program prog;
+    plan_forward(gates, n_gates, std::max(n, kRegBits), debug_options(1, tile_qubits, low_qubits, 0), prog);
+    std::vector<char> covered;
+    const std::string src = jit_source(prog, covered);
+    if (buf && buflen) {
+        const size_t m = std::min(buflen - 1, src.size());
+        std::memcpy(buf, src.data(), m);
+        buf[m] = 0;
+    }
+    if (compile) {
+        std::string err;
+        double ms = 0.0;
+        const size_t bytes = jit_compile_only(prog, "gfx950", &ms, err);
+        if (!bytes) return fail(SVMI_ERR_UNSUPPORTED, "jit: " + err);
+        if (compile_ms) *compile_ms = ms;
+        if (code_bytes) *code_bytes = bytes;
+    }
     return SVMI_OK;
 }
 

@@ -15,8 +15,16 @@
 // act on R only; diagonal ops may act on any qubit (tile bit or not).
 #pragma once
 
+#if defined(__HIPCC_RTC__)   // runtime compilation: its runtime header has the integer types
+using __hip_internal::int32_t;
+using __hip_internal::uint16_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+using __hip_internal::uint8_t;
+#else
 #include <stdint.h>
 #include <stddef.h>
+#endif
 
 #if defined(__HIPCC__)
 #define SVMI_HD __host__ __device__ __attribute__((always_inline))

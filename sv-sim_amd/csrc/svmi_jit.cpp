@@ -1,0 +1,274 @@
+// svmi_jit.cpp -- source generation, hipRTC compilation and launch of
+// specialised sweep kernels (see svmi_jit.hpp).
+#include "svmi_jit.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <sstream>
+
+namespace svmi {
+
+// The engine headers the generated source includes (embedded by build.py).
+extern const char* const kJitHeaderNames[];
+extern const char* const kJitHeaderText[];
+extern const int kJitHeaderCount;
+
+namespace {
+
+constexpr uint32_t kJitMaxTileQubits = 12;   // static LDS of one tile: 64 KiB
+
+void emit_hdr(std::ostringstream& o, const DOpHdr& h)
+{
+    o << "{" << h.type << "u, " << h.smask << "u, " << h.s0 << "u, " << h.s1 << "u, " << h.coef << "u, " << h.grad
+      << "u, " << h.rsel << "u, " << h.gmask << "u, " << h.emask << "ull}";
+}
+
+// Registers r (0..15) whose register bits satisfy `rsel`, as a list.
+std::vector<int> selected(uint32_t rsel)
+{
+    std::vector<int> v;
+    for (int r = 0; r < kRegs; r++)
+        if ((rsel >> r) & 1u) v.push_back(r);
+    return v;
+}
+
+// Tile-local offset of register r in phase ph (the register bits' part).
+uint32_t reg_offset(const PhaseInfo& ph, int r)
+{
+    uint32_t o = 0;
+    for (int b = 0; b < kRegBits; b++)
+        if ((r >> b) & 1) o |= 1u << ph.r[b];
+    return o;
+}
+
+// One op as direct code: the templates of svmi_exec.hpp / svmi_internal.hpp
+// with every slot, mask and offset a literal, so no dispatch is compiled.
+void emit_state_op(std::ostringstream& o, const DOp& d, const PhaseInfo& ph)
+{
+    const DOpHdr& h = d.h;
+    const std::string cond = "(base & " + std::to_string(h.emask) + "ull) == " + std::to_string(h.emask) +
+                             "ull && (goff & " + std::to_string(h.gmask) + "u) == " + std::to_string(h.gmask) + "u";
+    switch (h.type) {
+    case DOP_DENSE: o << "        apply_dense<" << h.smask << ">(a, coef + " << h.coef << ");\n"; return;
+    case DOP_X: o << "        apply_x<" << h.s0 << ">(a);\n"; return;
+    case DOP_CX: o << "        apply_cx<" << h.s0 << ", " << h.s1 << ">(a);\n"; return;
+    case DOP_SWAP: o << "        apply_swap<" << h.s0 << ", " << h.s1 << ">(a);\n"; return;
+    case DOP_SRN: o << "        apply_srn<" << h.s0 << ">(a);\n"; return;
+    case DOP_CDENSE:
+        o << "        if (" << cond << ") apply_cdense<" << h.s0 << ">(a, coef + " << h.coef << ", " << h.rsel
+          << "u);\n";
+        return;
+    case DOP_CPHASE: {
+        o << "        if (" << cond << ") {\n            const c64 ph = coef[" << h.coef << "];\n";
+        for (int r : selected(h.rsel)) o << "            a[" << r << "] = cmul(a[" << r << "], ph);\n";
+        o << "        }\n";
+        return;
+    }
+    case DOP_DIAG: {
+        // table index: bit i from qubit i -- outside the tile from the tile
+        // base, on a register bit a constant per register, else from goff
+        o << "        {\n            const uint32_t ix = 0u";
+        for (uint32_t i = 0; i < d.nq; i++) {
+            if (d.qloc[i] < 0) o << " | (uint32_t((base >> " << d.qphys[i] << ") & 1u) << " << i << ")";
+            else {
+                bool reg = false;
+                for (int b = 0; b < kRegBits; b++) reg = reg || int(ph.r[b]) == d.qloc[i];
+                if (!reg) o << " | (((goff >> " << d.qloc[i] << ") & 1u) << " << i << ")";
+            }
+        }
+        o << ";\n";
+        for (int r = 0; r < kRegs; r++) {
+            const uint32_t ro = reg_offset(ph, r);
+            uint32_t cidx = 0;
+            for (uint32_t i = 0; i < d.nq; i++)
+                if (d.qloc[i] >= 0 && ((ro >> d.qloc[i]) & 1u)) cidx |= 1u << i;
+            o << "            a[" << r << "] = cmul(a[" << r << "], coef[" << h.coef << "u + (ix | " << cidx
+              << "u)]);\n";
+        }
+        o << "        }\n";
+        return;
+    }
+    case DOP_PHASESET: {
+        o << "        {\n            const PhaseEntry* pe = reinterpret_cast<const PhaseEntry*>(coef + " << h.coef
+          << ");\n            double pr = " << (h.s1 ? "xf[" + std::to_string(h.grad) + "].re" : "1.0")
+          << ", pi = " << (h.s1 ? "xf[" + std::to_string(h.grad) + "].im" : "0.0") << ";\n"
+          << "#pragma unroll 4\n            for (uint32_t e = 0; e < " << h.s0 << "u; e++) {\n"
+          << "                const PhaseEntry E = pe[e];\n"
+          << "                if ((base & E.emask) != E.emask) continue;\n"
+          << "                const bool on = (goff & E.gmask) == E.gmask;\n"
+          << "                const double tr = SVMI_FMA(pr, E.ph_re, -(pi * E.ph_im));\n"
+          << "                const double ti = SVMI_FMA(pr, E.ph_im, pi * E.ph_re);\n"
+          << "                pr = on ? tr : pr;\n                pi = on ? ti : pi;\n            }\n"
+          << "            const c64 ph{pr, pi};\n";
+        for (int r : selected(h.rsel)) o << "            a[" << r << "] = cmul(a[" << r << "], ph);\n";
+        o << "        }\n";
+        return;
+    }
+    default: o << "        #error unsupported op type " << h.type << "\n"; return;
+    }
+}
+
+void emit_sweep(std::ostringstream& o, const Program& prog, uint32_t i)
+{
+    const SweepInfo& sw = prog.sweeps[i];
+    const uint32_t nt = 1u << (sw.k - kRegBits);
+    // second bound: at least two waves per SIMD, i.e. at most 256 VGPRs
+    o << "extern \"C\" __global__ void __launch_bounds__(" << nt << ", 2)\n"
+      << "svmi_jit_" << i << "(c64* __restrict__ s, const c64* __restrict__ coef, const uint64_t* __restrict__ ht, "
+         "uint64_t tile0)\n{\n"
+      << "    __shared__ __align__(16) c64 t0[" << (1u << sw.k) << "];\n"
+      << "    __shared__ __align__(16) c64 xf[" << (sw.n_xf ? sw.n_xf : 1u) << "];\n"
+      << "    const uint32_t tid = threadIdx.x;\n"
+      << "    const uint64_t tile = tile0 + blockIdx.x;\n"
+      << "    uint64_t base = 0;\n";
+    // tile base: the tile id's bits deposited on the qubits outside the tile
+    int j = 0;
+    for (int p = 0; p < 64; p++)
+        if ((sw.extmask >> p) & 1) o << "    base |= ((tile >> " << j++ << ") & 1ull) << " << p << ";\n";
+    const uint32_t cmask = (1u << sw.c) - 1u;
+    o << "#pragma unroll\n    for (int j = 0; j < " << kRegs << "; j++) {\n"
+      << "        const uint32_t l = tid + " << nt << "u * j;\n"
+      << "        t0[lds_slot(l)] = s[base | ht[" << sw.hitab << "u + (l >> " << sw.c << ")] | (l & " << cmask
+      << "u)];\n    }\n";
+    // tile-uniform phase factors
+    for (uint32_t op = sw.op_begin; op < sw.op_end; op++) {
+        const DOpHdr& h = prog.ops[op].h;
+        if (h.type == DOP_PHASESET && h.s1) {
+            o << "    if (tid == " << (h.grad % nt) << "u) {\n        constexpr DOpHdr H = ";
+            emit_hdr(o, h);
+            o << ";\n        xf[" << h.grad << "] = phase_factor(H, coef, base);\n    }\n";
+        }
+    }
+    o << "    __syncthreads();\n";
+    for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
+        const PhaseInfo& ph = prog.phases[p];
+        o << "    {\n        constexpr PhaseInfo P = {{" << ph.r[0] << "u, " << ph.r[1] << "u, " << ph.r[2] << "u, "
+          << ph.r[3] << "u}, 0u, 0u};\n"
+          << "        uint32_t off[kRegs];\n        phase_offsets(P, off);\n"
+          << "        const uint32_t goff = group_offset(P, tid);\n        c64 a[kRegs];\n"
+          << "#pragma unroll\n        for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(goff | off[r])];\n";
+        for (uint32_t op = ph.op_begin; op < ph.op_end; op++) emit_state_op(o, prog.ops[op], ph);
+        o << "#pragma unroll\n        for (int r = 0; r < kRegs; r++) t0[lds_slot(goff | off[r])] = a[r];\n"
+          << "    }\n    __syncthreads();\n";
+    }
+    o << "#pragma unroll\n    for (int j = 0; j < " << kRegs << "; j++) {\n"
+      << "        const uint32_t l = tid + " << nt << "u * j;\n"
+      << "        s[base | ht[" << sw.hitab << "u + (l >> " << sw.c << ")] | (l & " << cmask
+      << "u)] = t0[lds_slot(l)];\n    }\n}\n\n";
+}
+
+struct Rtc {
+    hiprtcProgram prog = nullptr;
+    ~Rtc()
+    {
+        if (prog) hiprtcDestroyProgram(&prog);
+    }
+};
+
+bool compile(const std::string& src, const char* arch, std::vector<char>& code, double* ms, std::string& err)
+{
+    Rtc r;
+    if (hiprtcCreateProgram(&r.prog, src.c_str(), "svmi_jit.hip", kJitHeaderCount,
+                            const_cast<const char**>(kJitHeaderText),
+                            const_cast<const char**>(kJitHeaderNames)) != HIPRTC_SUCCESS) {
+        err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const std::string a = std::string("--offload-arch=") + arch;
+    const char* opts[] = {a.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops", "-mllvm", "-structurizecfg-skip-uniform-regions=true"};
+    const auto t0 = std::chrono::steady_clock::now();
+    const hiprtcResult rc = hiprtcCompileProgram(r.prog, int(sizeof(opts) / sizeof(opts[0])), opts);
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(r.prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(r.prog, &log[0]);
+        err = std::string("hiprtc: ") + hiprtcGetErrorString(rc) + "\n" + log.substr(0, 2000);
+        return false;
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(r.prog, &n);
+    code.resize(n);
+    hiprtcGetCode(r.prog, code.data());
+    return n > 0;
+}
+
+}  // namespace
+
+void JitSweeps::release()
+{
+    if (module) hipModuleUnload(module);
+    module = nullptr;
+    fn.clear();
+}
+
+std::string jit_source(const Program& prog, std::vector<char>& covered)
+{
+    covered.assign(prog.sweeps.size(), 0);
+    std::ostringstream o;
+    o << "#include \"svmi_exec.hpp\"\nusing namespace svmi;\n\n";
+    if (prog.n_states != 1) return o.str();
+    for (uint32_t i = 0; i < prog.sweeps.size(); i++) {
+        const SweepInfo& sw = prog.sweeps[i];
+        if (sw.k > kJitMaxTileQubits) continue;
+        emit_sweep(o, prog, i);
+        covered[i] = 1;
+    }
+    return o.str();
+}
+
+size_t jit_compile_only(const Program& prog, const char* arch, double* ms, std::string& err)
+{
+    std::vector<char> covered, code;
+    const std::string src = jit_source(prog, covered);
+    if (!compile(src, arch, code, ms, err)) return 0;
+    return code.size();
+}
+
+bool jit_build(const Program& prog, const char* arch, JitSweeps& out, std::string& err)
+{
+    out.release();
+    std::vector<char> covered, code;
+    const std::string src = jit_source(prog, covered);
+    bool any = false;
+    for (char c : covered) any = any || c;
+    if (!any) return false;
+    if (!compile(src, arch, code, &out.compile_ms, err)) return false;
+    if (hipModuleLoadData(&out.module, code.data()) != hipSuccess) {
+        err = "hipModuleLoadData failed";
+        out.module = nullptr;
+        return false;
+    }
+    out.fn.assign(prog.sweeps.size(), nullptr);
+    for (uint32_t i = 0; i < prog.sweeps.size(); i++) {
+        if (!covered[i]) continue;
+        const std::string name = "svmi_jit_" + std::to_string(i);
+        if (hipModuleGetFunction(&out.fn[i], out.module, name.c_str()) != hipSuccess) {
+            err = "hipModuleGetFunction(" + name + ") failed";
+            out.release();
+            return false;
+        }
+    }
+    return true;
+}
+
+hipError_t jit_launch(hipStream_t st, const JitSweeps& j, uint32_t i, const SweepInfo& sw, c64* s, const c64* coef,
+                      const uint64_t* hitab)
+{
+    const uint32_t nt = 1u << (sw.k - kRegBits);
+    const uint64_t max_grid = 1ull << 30;
+    for (uint64_t t0 = 0; t0 < sw.n_tiles; t0 += max_grid) {
+        const uint32_t cnt = uint32_t(sw.n_tiles - t0 < max_grid ? sw.n_tiles - t0 : max_grid);
+        uint64_t tile0 = t0;
+        void* args[] = {&s, &coef, &hitab, &tile0};
+        const hipError_t e = hipModuleLaunchKernel(j.fn[i], cnt, 1, 1, nt, 1, 1, 0, st, args, nullptr);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace svmi

@@ -43,7 +43,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
           5: "ERR_DEVICE", 6: "ERR_OOM", 7: "ERR_UNSUPPORTED"}
 
 (OPT_FUSION, OPT_TILE_QUBITS, OPT_LOW_QUBITS, OPT_SWEEP_FLOPS, OPT_PROFILE, OPT_PIPELINE, OPT_OP_COST,
- OPT_COEF_BUDGET) = range(8)
+ OPT_COEF_BUDGET, OPT_SPECIALIZE) = range(9)
 
 # Every exported entry point (include/svmi.h, include/svmi_debug.h).
 EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state", "svmi_set_random_state",
@@ -51,7 +51,8 @@ EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
            "svmi_run", "svmi_measure", "svmi_expval", "svmi_adjoint", "svmi_batch_adjoint",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version", "svmi_qir_run", "svmi_qasm_parse",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
-           "svmi_debug_plan_batch", "svmi_debug_plan_adjoint", "svmi_debug_emulate_qir", "svmi_debug_plan_qir"]
+           "svmi_debug_plan_batch", "svmi_debug_plan_adjoint", "svmi_debug_emulate_qir", "svmi_debug_plan_qir",
+           "svmi_debug_jit"]
 
 
 class SvmiError(RuntimeError):
@@ -64,7 +65,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_uint64), ("kernel_launches", ctypes.c_uint64),
                 ("ops", ctypes.c_uint64), ("gates", ctypes.c_uint64), ("device_ms", ctypes.c_double),
                 ("sweep_launches", ctypes.c_uint64), ("sweep_ms", ctypes.c_double),
-                ("other_launches", ctypes.c_uint64), ("other_ms", ctypes.c_double)]
+                ("other_launches", ctypes.c_uint64), ("other_ms", ctypes.c_double),
+                ("specialized_sweeps", ctypes.c_uint64), ("compile_ms", ctypes.c_double)]
 
 
 class BatchItem(ctypes.Structure):
@@ -127,6 +129,9 @@ def lib():
         L.svmi_debug_plan_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
         L.svmi_qir_run.argtypes = [_vp, ctypes.c_int, _vp, _sz, _vp, _vp, _vp]
+        L.svmi_debug_jit.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
+                                     ctypes.c_char_p, _sz]
         L.svmi_debug_emulate_qir.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                              _vp]
         L.svmi_debug_plan_qir.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int,
@@ -434,3 +439,14 @@ def plan_qir(n, gates, tile_qubits=12, low_qubits=4):
     _check(lib().svmi_debug_plan_qir(int(n), _ptr(g), len(g), int(tile_qubits), int(low_qubits), ctypes.byref(sw),
                                      ctypes.byref(nm), buf, len(buf)))
     return sw.value, nm.value, buf.value.decode()
+
+
+def jit_source(n, gates, tile_qubits=12, low_qubits=4, compile=False):
+    """(source, compile_ms, code_bytes) of the specialised sweep kernels."""
+    g = as_gates(gates)
+    buf = ctypes.create_string_buffer(1 << 24)
+    ms = ctypes.c_double(0.0)
+    nb = ctypes.c_uint64(0)
+    _check(lib().svmi_debug_jit(int(n), _ptr(g), len(g), int(tile_qubits), int(low_qubits), int(compile),
+                                ctypes.byref(ms), ctypes.byref(nb), buf, len(buf)))
+    return buf.value.decode(), ms.value, nb.value

@@ -29,9 +29,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_struct_layout():
-    assert svmi.lib().svmi_abi_version() == 1
+    assert svmi.lib().svmi_abi_version() == 2
     assert svmi.GATE_DTYPE.itemsize == 48 and svmi.PAULI_DTYPE.itemsize == 24
-    assert ctypes.sizeof(svmi.BatchItem) == 56
+    assert ctypes.sizeof(svmi.BatchItem) == 56 and ctypes.sizeof(svmi.Stats) == 88
 
 
 def test_pybind_module_surface():

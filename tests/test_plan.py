@@ -153,3 +153,17 @@ def test_phase_factors_match_oracle(n, tile, low):
     assert "u" in "".join(tok for tok in desc.split() if tok.startswith("PS"))
     st = W.random_state(n, seed=n + 40)
     assert _err(svmi.emulate(n, gates, st, tile_qubits=tile, low_qubits=low), O.run(n, gates, st)) < 1e-12
+
+
+def test_specialised_sweep_source_compiles_without_a_device():
+    # hipRTC compiles for gfx950 on the host; tiles above 12 qubits get no kernel
+    rng = np.random.default_rng(31)
+    gates = []
+    for op in range(38):
+        k = svmi.ARITY[svmi.OP_NAMES[op]]
+        gates.append((op, list(rng.choice(9, size=k, replace=False)), *(rng.random(3) * 8 - 4)))
+    src, ms, nbytes = svmi.jit_source(9, gates, tile_qubits=8, low_qubits=2, compile=True)
+    assert src.count("__global__") >= 1 and nbytes > 0 and ms > 0
+    assert "svmi_jit_0" in src and "#error" not in src
+    src13, _, _ = svmi.jit_source(14, gates, tile_qubits=13, compile=False)
+    assert "__global__" not in src13
