@@ -103,7 +103,8 @@ typedef enum svmi_option {
     SVMI_OPT_PROFILE = 4,     /* 1 = time every launch with HIP events (svmi_stats)         */
     SVMI_OPT_PIPELINE = 5,    /* 1 = persistent sweeps prefetching the next tile,
                                  2 = the same with one 512-VGPR workgroup per CU, 0 = off (default) */
-    SVMI_OPT_OP_COST = 6,     /* fusion cost model: per-op cost in FMA per amplitude (default 8) */
+    SVMI_OPT_OP_COST = 6,     /* fusion cost model: per-op cost in FMA per amplitude (default 2;
+                                 0 for SVMI_OPT_SPECIALIZE unless set)                      */
     SVMI_OPT_COEF_BUDGET = 7, /* split a forward sweep whose op coefficients exceed this many
                                  bytes into consecutive sweeps (default 20000, 0 = never)   */
     SVMI_OPT_SPECIALIZE = 8   /* 1 = compile each uploaded circuit's sweeps into straight-line

@@ -27,11 +27,11 @@ constexpr int cpopc(int v) { return v ? (v & 1) + cpopc(v >> 1) : 0; }
 
 // kMaxQ: widest dense op compiled in (the planner emits no wider one for the
 // kernel instantiated with it)
-template <int kMaxQ>
-SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[kRegs], const c64* __restrict__ M)
+template <int kMaxQ, int N>
+SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[N], const c64* __restrict__ M)
 {
     switch (smask) {
-#define SVMI_DENSE_CASE(S) SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ) apply_dense<S>(a, M))
+#define SVMI_DENSE_CASE(S) SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ) (apply_dense<S, N>(a, M)))
         SVMI_DENSE_CASE(1) SVMI_DENSE_CASE(2) SVMI_DENSE_CASE(3) SVMI_DENSE_CASE(4)
         SVMI_DENSE_CASE(5) SVMI_DENSE_CASE(6) SVMI_DENSE_CASE(7) SVMI_DENSE_CASE(8)
         SVMI_DENSE_CASE(9) SVMI_DENSE_CASE(10) SVMI_DENSE_CASE(11) SVMI_DENSE_CASE(12)
@@ -42,12 +42,11 @@ SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[kRegs], const c64* __rest
 }
 
 template <int kMaxQ>
-SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&p)[kRegs], const c64 (&l)[kRegs],
-                                          const c64* __restrict__ M)
+SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&pl)[2 * kRegs], const c64* __restrict__ M)
 {
     double sum = 0.0;
     switch (smask) {
-#define SVMI_GRAD_CASE(S) SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ) sum = grad_dense<S>(p, l, M))
+#define SVMI_GRAD_CASE(S) SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ) sum = grad_dense<S>(pl, M))
         SVMI_GRAD_CASE(1) SVMI_GRAD_CASE(2) SVMI_GRAD_CASE(3) SVMI_GRAD_CASE(4)
         SVMI_GRAD_CASE(5) SVMI_GRAD_CASE(6) SVMI_GRAD_CASE(7) SVMI_GRAD_CASE(8)
         SVMI_GRAD_CASE(9) SVMI_GRAD_CASE(10) SVMI_GRAD_CASE(11) SVMI_GRAD_CASE(12)
@@ -58,71 +57,76 @@ SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&p)[kRegs], cons
     return sum;
 }
 
-SVMI_HD inline void dispatch_x(int s, c64 (&a)[kRegs])
+template <int N>
+SVMI_HD inline void dispatch_x(int s, c64 (&a)[N])
 {
     switch (s) {
-        SVMI_CASE(0, apply_x<0>(a))
-        SVMI_CASE(1, apply_x<1>(a))
-        SVMI_CASE(2, apply_x<2>(a))
-        SVMI_CASE(3, apply_x<3>(a))
+        SVMI_CASE(0, (apply_x<0, N>(a)))
+        SVMI_CASE(1, (apply_x<1, N>(a)))
+        SVMI_CASE(2, (apply_x<2, N>(a)))
+        SVMI_CASE(3, (apply_x<3, N>(a)))
     default: break;
     }
 }
 
-SVMI_HD inline void dispatch_cx(int c, int t, c64 (&a)[kRegs])
+template <int N>
+SVMI_HD inline void dispatch_cx(int c, int t, c64 (&a)[N])
 {
     switch (c * 4 + t) {
-        SVMI_CASE(1, (apply_cx<0, 1>(a)))
-        SVMI_CASE(2, (apply_cx<0, 2>(a)))
-        SVMI_CASE(3, (apply_cx<0, 3>(a)))
-        SVMI_CASE(4, (apply_cx<1, 0>(a)))
-        SVMI_CASE(6, (apply_cx<1, 2>(a)))
-        SVMI_CASE(7, (apply_cx<1, 3>(a)))
-        SVMI_CASE(8, (apply_cx<2, 0>(a)))
-        SVMI_CASE(9, (apply_cx<2, 1>(a)))
-        SVMI_CASE(11, (apply_cx<2, 3>(a)))
-        SVMI_CASE(12, (apply_cx<3, 0>(a)))
-        SVMI_CASE(13, (apply_cx<3, 1>(a)))
-        SVMI_CASE(14, (apply_cx<3, 2>(a)))
+        SVMI_CASE(1, (apply_cx<0, 1, N>(a)))
+        SVMI_CASE(2, (apply_cx<0, 2, N>(a)))
+        SVMI_CASE(3, (apply_cx<0, 3, N>(a)))
+        SVMI_CASE(4, (apply_cx<1, 0, N>(a)))
+        SVMI_CASE(6, (apply_cx<1, 2, N>(a)))
+        SVMI_CASE(7, (apply_cx<1, 3, N>(a)))
+        SVMI_CASE(8, (apply_cx<2, 0, N>(a)))
+        SVMI_CASE(9, (apply_cx<2, 1, N>(a)))
+        SVMI_CASE(11, (apply_cx<2, 3, N>(a)))
+        SVMI_CASE(12, (apply_cx<3, 0, N>(a)))
+        SVMI_CASE(13, (apply_cx<3, 1, N>(a)))
+        SVMI_CASE(14, (apply_cx<3, 2, N>(a)))
     default: break;
     }
 }
 
-SVMI_HD inline void dispatch_swap(int s0, int s1, c64 (&a)[kRegs])
+template <int N>
+SVMI_HD inline void dispatch_swap(int s0, int s1, c64 (&a)[N])
 {
     switch (s0 * 4 + s1) {
-        SVMI_CASE(1, (apply_swap<0, 1>(a)))
-        SVMI_CASE(2, (apply_swap<0, 2>(a)))
-        SVMI_CASE(3, (apply_swap<0, 3>(a)))
-        SVMI_CASE(6, (apply_swap<1, 2>(a)))
-        SVMI_CASE(7, (apply_swap<1, 3>(a)))
-        SVMI_CASE(11, (apply_swap<2, 3>(a)))
+        SVMI_CASE(1, (apply_swap<0, 1, N>(a)))
+        SVMI_CASE(2, (apply_swap<0, 2, N>(a)))
+        SVMI_CASE(3, (apply_swap<0, 3, N>(a)))
+        SVMI_CASE(6, (apply_swap<1, 2, N>(a)))
+        SVMI_CASE(7, (apply_swap<1, 3, N>(a)))
+        SVMI_CASE(11, (apply_swap<2, 3, N>(a)))
     default: break;
     }
 }
 
-SVMI_HD inline void dispatch_srn(int s, c64 (&a)[kRegs])
+template <int N>
+SVMI_HD inline void dispatch_srn(int s, c64 (&a)[N])
 {
     switch (s) {
-        SVMI_CASE(0, apply_srn<0>(a))
-        SVMI_CASE(1, apply_srn<1>(a))
-        SVMI_CASE(2, apply_srn<2>(a))
-        SVMI_CASE(3, apply_srn<3>(a))
+        SVMI_CASE(0, (apply_srn<0, N>(a)))
+        SVMI_CASE(1, (apply_srn<1, N>(a)))
+        SVMI_CASE(2, (apply_srn<2, N>(a)))
+        SVMI_CASE(3, (apply_srn<3, N>(a)))
     default: break;
     }
 }
 
-SVMI_HD inline void dispatch_cdense(const DOpHdr& h, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+template <int N>
+SVMI_HD inline void dispatch_cdense(const DOpHdr& h, c64 (&a)[N], const c64* __restrict__ coef, uint32_t goff,
                                     uint64_t base)
 {
     if ((base & h.emask) != h.emask) return;
     if ((goff & h.gmask) != h.gmask) return;
     const c64* M = coef + h.coef;
     switch (h.s0) {
-        SVMI_CASE_ID(0, 200, apply_cdense<0>(a, M, h.rsel))
-        SVMI_CASE_ID(1, 201, apply_cdense<1>(a, M, h.rsel))
-        SVMI_CASE_ID(2, 202, apply_cdense<2>(a, M, h.rsel))
-        SVMI_CASE_ID(3, 203, apply_cdense<3>(a, M, h.rsel))
+        SVMI_CASE_ID(0, 200, (apply_cdense<0, N>(a, M, h.rsel)))
+        SVMI_CASE_ID(1, 201, (apply_cdense<1, N>(a, M, h.rsel)))
+        SVMI_CASE_ID(2, 202, (apply_cdense<2, N>(a, M, h.rsel)))
+        SVMI_CASE_ID(3, 203, (apply_cdense<3, N>(a, M, h.rsel)))
     default: break;
     }
 }
@@ -153,7 +157,8 @@ SVMI_HD inline uint32_t diag_ext_index(const DOp& op, uint64_t base)
 // Controlled phase: amp *= ph where every qubit of the op is 1.  The
 // register part of the condition is the uniform mask rsel, the group part
 // one compare per thread, the part outside the tile one compare per tile.
-SVMI_HD inline void apply_cphase(const DOpHdr& op, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+template <int N>
+SVMI_HD inline void apply_cphase(const DOpHdr& op, c64 (&a)[N], const c64* __restrict__ coef, uint32_t goff,
                                  uint64_t base)
 {
     if ((base & op.emask) != op.emask) return;
@@ -161,24 +166,29 @@ SVMI_HD inline void apply_cphase(const DOpHdr& op, c64 (&a)[kRegs], const c64* _
     const c64 ph = coef[op.coef];
     const uint32_t rsel = op.rsel;
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
-        if ((rsel >> r) & 1u) a[r] = cmul(a[r], ph);
+    for (int r = 0; r < N; r++)
+        if ((rsel >> (r & (kRegs - 1))) & 1u) a[r] = cmul(a[r], ph);
 }
 
-SVMI_HD inline void apply_diag(const DOp& op, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+template <int N>
+SVMI_HD inline void apply_diag(const DOp& op, c64 (&a)[N], const c64* __restrict__ coef, uint32_t goff,
                                const uint32_t (&off)[kRegs], uint64_t base)
 {
     const uint32_t e = diag_ext_index(op, base);
     const c64* tab = coef + op.h.coef;
+    c64 f[kRegs];
 #pragma unroll
-    for (int r = 0; r < kRegs; r++) a[r] = cmul(a[r], tab[diag_index(op, goff | off[r], e)]);
+    for (int r = 0; r < kRegs; r++) f[r] = tab[diag_index(op, goff | off[r], e)];
+#pragma unroll
+    for (int r = 0; r < N; r++) a[r] = cmul(a[r], f[r & (kRegs - 1)]);
 }
 
 // Apply one non-gradient op to one state's register group.
 // goff = tile-local index bits of the group (non-register bits),
 // off[rho] = tile-local index bits of register rho, base = tile base index.
 // Product of controlled phases (h.s0 entries) applied to registers h.rsel.
-SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[kRegs], const c64* __restrict__ coef, uint32_t goff,
+template <int N>
+SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[N], const c64* __restrict__ coef, uint32_t goff,
                                    uint64_t base, const c64* xf)
 {
     const PhaseEntry* pe = reinterpret_cast<const PhaseEntry*>(coef + h.coef);
@@ -202,8 +212,8 @@ SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[kRegs], const c64* 
     const c64 ph{pr, pi};
     const uint32_t rsel = h.rsel;
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
-        if ((rsel >> r) & 1u) a[r] = cmul(a[r], ph);
+    for (int r = 0; r < N; r++)
+        if ((rsel >> (r & (kRegs - 1))) & 1u) a[r] = cmul(a[r], ph);
 }
 
 // Factor slot of a DOP_PHASESET op for the tile at `base`: the product of
@@ -224,40 +234,41 @@ SVMI_HD inline c64 phase_factor(const DOpHdr& h, const c64* __restrict__ coef, u
 }
 
 // h = the op's header (prefetched), op = the op in memory (diagonal tail).
-template <int kMaxQ = kRegBits>
-SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[kRegs],
+template <int kMaxQ = kRegBits, int N = kRegs>
+SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[N],
                                   const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
                                   uint64_t base, const c64* xf)
 {
     switch (h.type) {
-        SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, dispatch_dense<kMaxQ>(h.smask, a, coef + h.coef))
-        SVMI_CASE_ID(DOP_X, DOP_X + 100, dispatch_x(h.s0, a))
-        SVMI_CASE_ID(DOP_CX, DOP_CX + 100, dispatch_cx(h.s0, h.s1, a))
-        SVMI_CASE_ID(DOP_SWAP, DOP_SWAP + 100, dispatch_swap(h.s0, h.s1, a))
-        SVMI_CASE_ID(DOP_SRN, DOP_SRN + 100, dispatch_srn(h.s0, a))
-        SVMI_CASE_ID(DOP_CPHASE, DOP_CPHASE + 100, apply_cphase(h, a, coef, goff, base))
-        SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag(*op, a, coef, goff, off, base))
-        SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset(h, a, coef, goff, base, xf))
-        SVMI_CASE_ID(DOP_CDENSE, DOP_CDENSE + 100, dispatch_cdense(h, a, coef, goff, base))
+        SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, (dispatch_dense<kMaxQ, N>(h.smask, a, coef + h.coef)))
+        SVMI_CASE_ID(DOP_X, DOP_X + 100, dispatch_x<N>(h.s0, a))
+        SVMI_CASE_ID(DOP_CX, DOP_CX + 100, dispatch_cx<N>(h.s0, h.s1, a))
+        SVMI_CASE_ID(DOP_SWAP, DOP_SWAP + 100, dispatch_swap<N>(h.s0, h.s1, a))
+        SVMI_CASE_ID(DOP_SRN, DOP_SRN + 100, dispatch_srn<N>(h.s0, a))
+        SVMI_CASE_ID(DOP_CPHASE, DOP_CPHASE + 100, apply_cphase<N>(h, a, coef, goff, base))
+        SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag<N>(*op, a, coef, goff, off, base))
+        SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset<N>(h, a, coef, goff, base, xf))
+        SVMI_CASE_ID(DOP_CDENSE, DOP_CDENSE + 100, dispatch_cdense<N>(h, a, coef, goff, base))
     default: break;
     }
 }
 
 // Gradient op: Re sum conj(lam) K phi over this group.
 template <int kMaxQ = kRegBits>
-SVMI_HD inline double exec_grad_op(const DOpHdr& h, const DOp* __restrict__ op, const c64 (&p)[kRegs],
-                                   const c64 (&l)[kRegs], const c64* __restrict__ coef, uint32_t goff,
-                                   const uint32_t (&off)[kRegs], uint64_t base)
+// pl = phi's register group then lambda's.
+SVMI_HD inline double exec_grad_op(const DOpHdr& h, const DOp* __restrict__ op, const c64 (&pl)[2 * kRegs],
+                                   const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
+                                   uint64_t base)
 {
-    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense<kMaxQ>(h.smask, p, l, coef + h.coef);
+    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense<kMaxQ>(h.smask, pl, coef + h.coef);
     // DOP_GRAD_DIAG
     const uint32_t e = diag_ext_index(*op, base);
     const c64* tab = coef + h.coef;
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < kRegs; r++) {
-        const c64 v = cmul(tab[diag_index(*op, goff | off[r], e)], p[r]);
-        s += l[r].re * v.re + l[r].im * v.im;
+        const c64 v = cmul(tab[diag_index(*op, goff | off[r], e)], pl[r]);
+        s += pl[kRegs + r].re * v.re + pl[kRegs + r].im * v.im;
     }
     return s;
 }

@@ -163,6 +163,9 @@ struct StandaloneOp {
 // ----------------------------------------------------------------------------
 // Op application on one register group (shared by kernels and host checker).
 // a[rho], rho = 4 register bits; bit s of rho <-> register slot s <-> tile bit r[s].
+// N = 2 * kRegs holds the same group of two states (phi in a[0..15], lambda
+// in a[16..31]): slot bits never reach bit 4, so one call applies the op to
+// both states and the op is dispatched once.
 // ----------------------------------------------------------------------------
 
 template <int S>
@@ -204,29 +207,32 @@ struct SlotMask {
 };
 
 // Dense 2^q x 2^q (row-major, matrix bit i <-> i-th set slot of S).
-template <int S>
-SVMI_HD inline void apply_dense(c64 (&a)[kRegs], const c64* __restrict__ M)
+template <int S, int N = kRegs>
+SVMI_HD inline void apply_dense(c64 (&a)[N], const c64* __restrict__ M)
 {
     constexpr int q = SlotMask<S>::q;
     constexpr int D = 1 << q;
 #pragma unroll
-    for (int o = 0; o < (kRegs >> q); o++) {
-        c64 v[D];
+    for (int h = 0; h < N; h += kRegs)
 #pragma unroll
-        for (int j = 0; j < D; j++) v[j] = a[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)];
+        for (int o = 0; o < (kRegs >> q); o++) {
+            c64 v[D];
 #pragma unroll
-        for (int i = 0; i < D; i++) {
-            c64 acc = c64{0.0, 0.0};
+            for (int j = 0; j < D; j++) v[j] = a[h | SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)];
 #pragma unroll
-            for (int j = 0; j < D; j++) acc = cmac(acc, M[i * D + j], v[j]);
-            a[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i)] = acc;
+            for (int i = 0; i < D; i++) {
+                c64 acc = c64{0.0, 0.0};
+#pragma unroll
+                for (int j = 0; j < D; j++) acc = cmac(acc, M[i * D + j], v[j]);
+                a[h | SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i)] = acc;
+            }
         }
-    }
 }
 
-// Re sum_groups conj(l) . (M p) over the slots of S (state untouched).
+// Re sum_groups conj(l) . (M p) over the slots of S (state untouched);
+// pl = phi's group then lambda's.
 template <int S>
-SVMI_HD inline double grad_dense(const c64 (&p)[kRegs], const c64 (&l)[kRegs], const c64* __restrict__ M)
+SVMI_HD inline double grad_dense(const c64 (&pl)[2 * kRegs], const c64* __restrict__ M)
 {
     constexpr int q = SlotMask<S>::q;
     constexpr int D = 1 << q;
@@ -238,19 +244,19 @@ SVMI_HD inline double grad_dense(const c64 (&p)[kRegs], const c64 (&l)[kRegs], c
             c64 acc = c64{0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < D; j++)
-                acc = cmac(acc, M[i * D + j], p[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)]);
-            const c64 lv = l[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i)];
+                acc = cmac(acc, M[i * D + j], pl[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)]);
+            const c64 lv = pl[kRegs + (SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i))];
             sum += lv.re * acc.re + lv.im * acc.im;
         }
     }
     return sum;
 }
 
-template <int s>
-SVMI_HD inline void apply_x(c64 (&a)[kRegs])
+template <int s, int N = kRegs>
+SVMI_HD inline void apply_x(c64 (&a)[N])
 {
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
+    for (int r = 0; r < N; r++)
         if (!((r >> s) & 1)) {
             const c64 t = a[r];
             a[r] = a[r | (1 << s)];
@@ -258,11 +264,11 @@ SVMI_HD inline void apply_x(c64 (&a)[kRegs])
         }
 }
 
-template <int c, int t>
-SVMI_HD inline void apply_cx(c64 (&a)[kRegs])
+template <int c, int t, int N = kRegs>
+SVMI_HD inline void apply_cx(c64 (&a)[N])
 {
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
+    for (int r = 0; r < N; r++)
         if (((r >> c) & 1) && !((r >> t) & 1)) {
             const c64 tmp = a[r];
             a[r] = a[r | (1 << t)];
@@ -270,11 +276,11 @@ SVMI_HD inline void apply_cx(c64 (&a)[kRegs])
         }
 }
 
-template <int s0, int s1>
-SVMI_HD inline void apply_swap(c64 (&a)[kRegs])
+template <int s0, int s1, int N = kRegs>
+SVMI_HD inline void apply_swap(c64 (&a)[N])
 {
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
+    for (int r = 0; r < N; r++)
         if (((r >> s0) & 1) && !((r >> s1) & 1)) {
             const int r2 = r ^ (1 << s0) ^ (1 << s1);
             const c64 tmp = a[r];
@@ -286,13 +292,13 @@ SVMI_HD inline void apply_swap(c64 (&a)[kRegs])
 // Controlled 2x2 (row-major M) on register slot s: the pairs (r, r | 1<<s)
 // whose register r is in rsel (the register part of the control condition;
 // rsel never tells r from r | 1<<s apart).
-template <int s>
-SVMI_HD inline void apply_cdense(c64 (&a)[kRegs], const c64* __restrict__ M, uint32_t rsel)
+template <int s, int N = kRegs>
+SVMI_HD inline void apply_cdense(c64 (&a)[N], const c64* __restrict__ M, uint32_t rsel)
 {
     const c64 m00 = M[0], m01 = M[1], m10 = M[2], m11 = M[3];
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
-        if (!((r >> s) & 1) && ((rsel >> r) & 1u)) {
+    for (int r = 0; r < N; r++)
+        if (!((r >> s) & 1) && ((rsel >> (r & (kRegs - 1))) & 1u)) {
             const c64 a0 = a[r], a1 = a[r | (1 << s)];
             a[r] = cmac(cmac(c64{0.0, 0.0}, m00, a0), m01, a1);
             a[r | (1 << s)] = cmac(cmac(c64{0.0, 0.0}, m10, a0), m11, a1);
@@ -300,11 +306,11 @@ SVMI_HD inline void apply_cdense(c64 (&a)[kRegs], const c64* __restrict__ M, uin
 }
 
 // Reference SRN_GATE (svsim_cpu_omp.hpp:866-879), as coded there.
-template <int s>
-SVMI_HD inline void apply_srn(c64 (&a)[kRegs])
+template <int s, int N = kRegs>
+SVMI_HD inline void apply_srn(c64 (&a)[N])
 {
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
+    for (int r = 0; r < N; r++)
         if (!((r >> s) & 1)) {
             const c64 a0 = a[r], a1 = a[r | (1 << s)];
             a[r] = c64{0.5 * (a0.re + a1.re), 0.5 * (a0.im - a1.im)};

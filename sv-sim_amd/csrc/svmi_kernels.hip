@@ -78,12 +78,13 @@ tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
             uint32_t off[kRegs];
             phase_offsets(ph, off);
             const uint32_t goff = group_offset(ph, tid);
-            c64 a[kRegs];
-            c64 b[kRegs];
+            // NS = 2: phi's group in a[0..15], lambda's in a[16..31]; each
+            // op is dispatched once for both
+            c64 a[NS * kRegs];
 #pragma unroll
             for (int r = 0; r < kRegs; r++) {
                 a[r] = t0[lds_slot(goff | off[r])];
-                if (NS == 2) b[r] = t1[lds_slot(goff | off[r])];
+                if constexpr (NS == 2) a[kRegs + r] = t1[lds_slot(goff | off[r])];
             }
             // op headers are fetched one op ahead so the scalar-load latency
             // of the next op overlaps the arithmetic of the current one
@@ -92,20 +93,22 @@ tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
             for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                 const DOpHdr h = nxt;
                 if (o + 1 < ph.op_end) nxt = ops[o + 1].h;
-                if (NS == 2 && (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG)) {
-                    double v = exec_grad_op<kMaxDenseQubitsAdj>(h, ops + o, a, b, coef, goff, off, base);
-                    const int width = nt < 64 ? int(nt) : 64;
-                    for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
-                    if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
-                } else {
-                    exec_state_op<NS == 2 ? kMaxDenseQubitsAdj : kRegBits>(h, ops + o, a, coef, goff, off, base, xf);
-                    if (NS == 2) exec_state_op<kMaxDenseQubitsAdj>(h, ops + o, b, coef, goff, off, base, xf);
+                if constexpr (NS == 2) {
+                    if (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG) {
+                        double v = exec_grad_op<kMaxDenseQubitsAdj>(h, ops + o, a, coef, goff, off, base);
+                        const int width = nt < 64 ? int(nt) : 64;
+                        for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
+                        if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
+                        continue;
+                    }
                 }
+                exec_state_op<NS == 2 ? kMaxDenseQubitsAdj : kRegBits, NS * kRegs>(h, ops + o, a, coef, goff, off,
+                                                                                  base, xf);
             }
 #pragma unroll
             for (int r = 0; r < kRegs; r++) {
                 t0[lds_slot(goff | off[r])] = a[r];
-                if (NS == 2) t1[lds_slot(goff | off[r])] = b[r];
+                if constexpr (NS == 2) t1[lds_slot(goff | off[r])] = a[kRegs + r];
             }
         }
         __syncthreads();

@@ -14,7 +14,7 @@ struct PlanOptions {
     int tile_qubits = 12;      // k: qubits per LDS tile (13 allowed: one 128 KiB tile per CU)
     int low_qubits = 4;        // c: physical qubits 0..c-1 in every tile
     long sweep_budget = 0;     // per-amplitude FMA budget of one sweep (0 = none)
-    long op_overhead = 8;      // fusion cost model: per-op cost on top of its FMAs (per amplitude)
+    long op_overhead = 2;      // fusion cost model: per-op cost on top of its FMAs (per amplitude)
     long coef_budget = 20000;  // split a sweep whose op coefficients exceed this many bytes (0 = never)
 };
 
