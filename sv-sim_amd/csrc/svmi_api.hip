@@ -640,6 +640,10 @@ svmi_status use_device(svmi_handle h)
     return SVMI_OK;
 }
 
+// largest gradient-partial matrix kept for one reduction at the end of a
+// two-state program (beyond it each sweep reduces its own partials)
+constexpr size_t kCombinedPartialsBytes = size_t(256) << 20;
+
 svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, c64* s0, c64* s1,
                          double* grad_sums /* device, prog.n_grad */,
                          MeasureResult* meas = nullptr /* device, prog.measures.size() */,
@@ -653,13 +657,28 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
             h->prof_events.push_back(e);
         }
     }
+    // Two-state programs: when every sweep with gradient ops has the same
+    // tile count (one tile size per program), all their partials go into one
+    // grad-major matrix [prog.n_grad][tiles] and one k_sum_rows launch after
+    // the last sweep reduces it -- the same per-row sums as a launch per sweep.
+    uint64_t common_tiles = 0;
+    bool combined = prog.n_states == 2 && prog.n_grad && grad_sums;
+    for (const SweepInfo& sw : prog.sweeps) {
+        if (!combined || !sw.n_grad) continue;
+        if (!common_tiles) common_tiles = sw.n_tiles;
+        combined = sw.n_tiles == common_tiles;
+    }
+    combined = combined && common_tiles && prog.n_grad * common_tiles * sizeof(double) <= kCombinedPartialsBytes;
+    if (combined) SVMI_TRY(h->scratch[5].ensure(prog.n_grad * common_tiles * sizeof(double)));
     size_t si = 0;
     for (const Step& st : prog.steps) {
         if (prof) SVMI_HIP(hipEventRecord(h->prof_events[2 * si], h->stream));
         if (st.kind == Step::SWEEP) {
             const SweepInfo& sw = prog.sweeps[st.index];
             double* partials = nullptr;
-            if (prog.n_states == 2 && sw.n_grad) {
+            if (combined && sw.n_grad) {
+                partials = h->scratch[5].as<double>() + sw.grad_base * common_tiles;
+            } else if (prog.n_states == 2 && sw.n_grad) {
                 SVMI_TRY(h->scratch[5].ensure(sw.n_tiles * sw.n_grad * sizeof(double)));
                 partials = h->scratch[5].as<double>();
             }
@@ -669,7 +688,7 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
                 SVMI_HIP(launch_sweep(h->stream, prog.n_states, s0, s1, sw, dp.phases.as<PhaseInfo>(),
                                       dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials,
                                       h->pipeline));
-            if (partials)
+            if (partials && !combined)
                 SVMI_HIP(launch_sum_rows(h->stream, partials, sw.n_grad, sw.n_tiles, grad_sums + sw.grad_base));
         } else if (st.kind == Step::MEASURE) {
             const MeasureStep& m = prog.measures[st.index];
@@ -685,6 +704,7 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
         h->stats.kernel_launches++;
         si++;
     }
+    if (combined) SVMI_HIP(launch_sum_rows(h->stream, h->scratch[5].as<double>(), prog.n_grad, common_tiles, grad_sums));
     h->stats.sweeps += prog.steps.size();
     if (prof) {
         SVMI_HIP(hipStreamSynchronize(h->stream));
@@ -704,11 +724,12 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
 }
 
 // <psi|H|psi>, optionally writing lam = H psi.
-svmi_status pauli_device(svmi_handle h, const c64* psi, c64* lam, const svmi_pauli_term* terms, size_t n_terms,
-                         double* out)
+// lam = H psi and the block partials of <psi|lam> in scratch[2] (grid of
+// them, returned); groups / pt hold the uploaded tables and must outlive the
+// stream's copies (the caller synchronises).
+svmi_status pauli_launch(svmi_handle h, const c64* psi, c64* lam, const svmi_pauli_term* terms, size_t n_terms,
+                         std::vector<PauliGroup>& groups, std::vector<PauliTerm>& pt, uint32_t* grid_out)
 {
-    std::vector<PauliGroup> groups;
-    std::vector<PauliTerm> pt;
     group_terms(terms, n_terms, groups, pt);
     const uint32_t grid = pauli_grid(h->dim);
     SVMI_TRY(h->scratch[0].ensure(std::max<size_t>(groups.size(), 1) * sizeof(PauliGroup)));
@@ -722,6 +743,17 @@ svmi_status pauli_device(svmi_handle h, const c64* psi, c64* lam, const svmi_pau
     }
     SVMI_HIP(launch_pauli(h->stream, psi, lam, h->dim, h->scratch[0].as<PauliGroup>(), uint32_t(groups.size()),
                           h->scratch[1].as<PauliTerm>(), h->scratch[2].as<double>()));
+    *grid_out = grid;
+    return SVMI_OK;
+}
+
+svmi_status pauli_device(svmi_handle h, const c64* psi, c64* lam, const svmi_pauli_term* terms, size_t n_terms,
+                         double* out)
+{
+    std::vector<PauliGroup> groups;
+    std::vector<PauliTerm> pt;
+    uint32_t grid = 0;
+    SVMI_TRY(pauli_launch(h, psi, lam, terms, n_terms, groups, pt, &grid));
     std::vector<double> part(grid);
     SVMI_HIP(hipMemcpyAsync(part.data(), h->scratch[2].p, grid * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     SVMI_HIP(hipStreamSynchronize(h->stream));
@@ -1118,17 +1150,20 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
     if (!expval_out || (n_params && !grad_out)) return fail(SVMI_ERR_ARG, "null output");
     SVMI_TRY(check_terms(terms, n_terms, h->n));
     SVMI_TRY(check_params(h->gates.data(), h->gates.size(), param_slots, n_params));
-    // forward
-    SVMI_TRY(svmi_run(h));
-    // phi = psi, lam = H psi, E = <psi|lam>
+    // the device work of the whole gradient is queued without host
+    // synchronisation; one copy-back and one sync at the end
     SVMI_TRY(h->aux[0].ensure(h->dim * sizeof(c64)));
     SVMI_TRY(h->aux[1].ensure(h->dim * sizeof(c64)));
+    // forward
+    SVMI_TRY(exec_program(h, h->prog, h->dprog, h->sv, nullptr, nullptr, nullptr, &h->jit));
+    // phi = psi, lam = H psi, partials of E = <psi|lam>
     c64* phi = h->aux[0].as<c64>();
     c64* lam = h->aux[1].as<c64>();
     SVMI_HIP(launch_copy(h->stream, phi, h->sv, h->dim));
-    double E = 0.0;
-    SVMI_TRY(pauli_device(h, h->sv, lam, terms, n_terms, &E));
-    *expval_out = E;
+    std::vector<PauliGroup> groups;
+    std::vector<PauliTerm> pt;
+    uint32_t grid = 0;
+    SVMI_TRY(pauli_launch(h, h->sv, lam, terms, n_terms, groups, pt, &grid));
     for (size_t p = 0; p < n_params; p++) grad_out[p] = 0.0;
     // backward (planned once per circuit and parameter map)
     const size_t n_slots = 3 * h->gates.size();
@@ -1147,11 +1182,15 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
     }
     const Program& bp = h->adj_prog;
     SVMI_TRY(exec_program(h, bp, h->adj_dprog, phi, lam, h->adj_gsum.as<double>()));
-    std::vector<double> g(bp.n_grad);
+    std::vector<double> g(bp.n_grad), part(grid);
     if (bp.n_grad)
         SVMI_HIP(hipMemcpyAsync(g.data(), h->adj_gsum.p, bp.n_grad * sizeof(double), hipMemcpyDeviceToHost,
                                 h->stream));
+    SVMI_HIP(hipMemcpyAsync(part.data(), h->scratch[2].p, grid * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     SVMI_HIP(hipStreamSynchronize(h->stream));
+    double E = 0.0;
+    for (double v : part) E += v;
+    *expval_out = E;
     for (uint32_t i = 0; i < bp.n_grad; i++) grad_out[bp.grad_param[i]] += 2.0 * g[i];
     return SVMI_OK;
 }

@@ -350,8 +350,13 @@ long reg_cost(const LOp& op)
     return 4L << op.k;
 }
 
-// matrix of `op` on the slot set S (matrix bit j <-> j-th lowest slot of S)
-void embed_in_slots(const LOp& op, const int* tloc, const int* slot_of, uint32_t S, std::vector<c64>& M)
+// T = E B where E is `op` embedded in the slot set S (matrix bit j <-> j-th
+// lowest slot of S) and B is D x D, D = 2^|S|.  Row r of E has nonzeros only
+// in the columns that differ from r in the op's bits, so each entry of T is a
+// sum over those 2^k columns, taken in increasing column order -- the order a
+// full D x D product would add the same (nonzero) terms in.
+void left_multiply_in_slots(const LOp& op, const int* tloc, const int* slot_of, uint32_t S,
+                            const std::vector<c64>& B, std::vector<c64>& T)
 {
     int pos_of_slot[kRegBits];
     int q = 0;
@@ -359,25 +364,37 @@ void embed_in_slots(const LOp& op, const int* tloc, const int* slot_of, uint32_t
         if ((S >> s) & 1) pos_of_slot[s] = q++;
     const size_t D = size_t(1) << q;
     int bitpos[5];
-    uint32_t opmask = 0;
+    size_t opmask = 0;
     for (int i = 0; i < op.k; i++) {
         bitpos[i] = pos_of_slot[slot_of[tloc[op.q[i]]]];
-        opmask |= 1u << bitpos[i];
+        opmask |= size_t(1) << bitpos[i];
     }
     const size_t Dop = size_t(1) << op.k;
     std::vector<c64> A;
     lop_to_dense(op, A);
-    M.assign(D * D, c64{0.0, 0.0});
-    for (size_t c = 0; c < D; c++) {
-        size_t cl = 0;
+    // op-local column index cl -> offset in the D-space, in increasing offset order
+    size_t order[32], offs[32];
+    for (size_t cl = 0; cl < Dop; cl++) {
+        size_t o = 0;
         for (int i = 0; i < op.k; i++)
-            if ((c >> bitpos[i]) & 1) cl |= size_t(1) << i;
-        const size_t rest = c & ~size_t(opmask);
-        for (size_t rl = 0; rl < Dop; rl++) {
-            size_t r = rest;
-            for (int i = 0; i < op.k; i++)
-                if ((rl >> i) & 1) r |= size_t(1) << bitpos[i];
-            M[r * D + c] = A[rl * Dop + cl];
+            if ((cl >> i) & 1) o |= size_t(1) << bitpos[i];
+        offs[cl] = o;
+        order[cl] = cl;
+    }
+    std::sort(order, order + Dop, [&](size_t x, size_t y) { return offs[x] < offs[y]; });
+    T.assign(D * D, c64{0.0, 0.0});
+    for (size_t r = 0; r < D; r++) {
+        size_t rl = 0;
+        for (int i = 0; i < op.k; i++)
+            if ((r >> bitpos[i]) & 1) rl |= size_t(1) << i;
+        const size_t rest = r & ~opmask;
+        for (size_t c = 0; c < D; c++) {
+            c64 acc{0.0, 0.0};
+            for (size_t t = 0; t < Dop; t++) {
+                const size_t cl = order[t];
+                acc = cmac(acc, A[rl * Dop + cl], B[(rest | offs[cl]) * D + c]);
+            }
+            T[r * D + c] = acc;
         }
     }
 }
@@ -423,16 +440,10 @@ struct RunMerger {
             for (const SOp* s : run) out.push_back(s);
         } else {
             const size_t D = size_t(1) << q;
-            std::vector<c64> B(D * D, c64{0.0, 0.0}), E, T(D * D);
+            std::vector<c64> B(D * D, c64{0.0, 0.0}), T;
             for (size_t i = 0; i < D; i++) B[i * D + i] = c64{1.0, 0.0};
             for (const SOp* s : run) {
-                embed_in_slots(*s->op, tloc, slot_of, S, E);
-                for (size_t r = 0; r < D; r++)
-                    for (size_t c = 0; c < D; c++) {
-                        c64 acc{0.0, 0.0};
-                        for (size_t j = 0; j < D; j++) acc = cmac(acc, E[r * D + j], B[j * D + c]);
-                        T[r * D + c] = acc;
-                    }
+                left_multiply_in_slots(*s->op, tloc, slot_of, S, B, T);
                 B.swap(T);
             }
             LOp m;
@@ -531,35 +542,43 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
 
     // phase scheduling: pick the register bits R of each phase by a small
     // search that maximises the work the phase can take, then take it
-    std::vector<const SOp*> remaining = taken;
-    uint32_t n_grad_sweep = 0;
-    auto local_mask = [&](const SOp* s) {
-        uint32_t lm = 0;
+    struct Pending {
+        const SOp* s;
+        uint32_t lm;      // tile-local bits the op touches
+        bool local;       // every qubit of the op is in the tile
+    };
+    std::vector<Pending> remaining;
+    remaining.reserve(taken.size());
+    uint64_t dep_all = 0;
+    for (const SOp* s : taken) {
+        Pending p{s, 0u, true};
         for (int i = 0; i < s->op->k; i++) {
             const int l = tloc[s->op->q[i]];
-            if (l >= 0) lm |= 1u << l;
+            if (l >= 0) p.lm |= 1u << l;
+            else p.local = false;
         }
-        return lm;
-    };
-    auto all_local = [&](const SOp* s) {
-        for (int i = 0; i < s->op->k; i++)
-            if (tloc[s->op->q[i]] < 0) return false;
-        return true;
-    };
-    // ops a phase with fixed register bits R takes; returns a score
-    auto take_with = [&](uint32_t R, std::vector<const SOp*>* ph, std::vector<const SOp*>* rest) {
+        dep_all |= s->dep;
+        remaining.push_back(p);
+    }
+    uint32_t n_grad_sweep = 0;
+    // ops a phase with fixed register bits R takes; returns a score.  Without
+    // output lists the scan stops once every qubit is blocked (nothing later
+    // can be taken or change the score).
+    auto take_with = [&](uint32_t R, std::vector<const SOp*>* ph, std::vector<Pending>* rest) {
         uint64_t blocked = 0;
         long score = 0;
-        for (const SOp* s : remaining) {
+        for (const Pending& p : remaining) {
+            const SOp* s = p.s;
             if (s->dep & blocked) {
                 blocked |= s->dep;
-                if (rest) rest->push_back(s);
+                if (rest) rest->push_back(p);
+                else if (blocked == dep_all) break;
                 continue;
             }
-            const uint32_t lm = local_mask(s);
+            const uint32_t lm = p.lm;
             if (s->diag) {
                 if (ph) ph->push_back(s);
-                if (all_local(s) && (lm & ~R) == 0) score += 1;          // merges into a run
+                if (p.local && (lm & ~R) == 0) score += 1;               // merges into a run
                 else if (lm & R) score -= 1;                             // splits a run
                 continue;
             }
@@ -569,21 +588,24 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                                                                          // taken saves a phase later)
             } else {
                 blocked |= s->dep;
-                if (rest) rest->push_back(s);
+                if (rest) rest->push_back(p);
+                else if (blocked == dep_all) break;
             }
         }
         return score;
     };
     while (!remaining.empty()) {
+        dep_all = 0;
+        for (const Pending& p : remaining) dep_all |= p.s->dep;
         uint32_t R = 0;  // local-bit mask
-        for (const SOp* s : remaining)
-            if (!s->diag) {
-                R = local_mask(s);
+        for (const Pending& p : remaining)
+            if (!p.s->diag) {
+                R = p.lm;
                 break;
             }
         // candidate bits: tile bits of the first pending ops
         uint32_t cand = 0;
-        for (size_t i = 0; i < remaining.size() && i < 96; i++) cand |= local_mask(remaining[i]);
+        for (size_t i = 0; i < remaining.size() && i < 96; i++) cand |= remaining[i].lm;
         while (popc(R) < kRegBits) {
             long best = -(1L << 60);
             int best_b = -1;
@@ -598,7 +620,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             if (best_b < 0) break;
             R |= 1u << best_b;
         }
-        std::vector<const SOp*> ph, rest;
+        std::vector<const SOp*> ph;
+        std::vector<Pending> rest;
         take_with(R, &ph, &rest);
         // pad R to 4 bits with the lowest free tile bits above the low qubits first
         for (int l = 0; l < k && popc(R) < kRegBits; l++)
