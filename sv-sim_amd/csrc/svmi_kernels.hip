@@ -59,6 +59,31 @@ tile_store(const c64* t, c64* __restrict__ s, const SweepInfo& sw, const uint64_
     }
 }
 
+// v of a lane moved by one DPP pattern (both halves of the double from the
+// same source lane); lanes without a source, or outside row_mask, get 0.
+template <int kCtrl, int kRowMask>
+__device__ __attribute__((always_inline)) inline double dpp_move(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, kRowMask, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, kRowMask, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum of v over the 64 lanes of a wave, left in lane 63: an inclusive scan
+// inside each 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1
+// and row 2's into row 3 (row_bcast:15), then rows 0+1 into rows 2-3
+// (row_bcast:31).  A fixed order, so the sum is the same on every run.
+__device__ __attribute__((always_inline)) inline double wave_sum_lane63(double v)
+{
+    v += dpp_move<0x111, 0xf>(v);
+    v += dpp_move<0x112, 0xf>(v);
+    v += dpp_move<0x114, 0xf>(v);
+    v += dpp_move<0x118, 0xf>(v);
+    v += dpp_move<0x142, 0xa>(v);
+    v += dpp_move<0x143, 0xc>(v);
+    return v;
+}
+
 // The phases of one sweep on the tile in LDS (t0, and t1 for NS = 2).
 // Threads tid < nt own one register group per phase; with kIdle (batched
 // kernel: blockDim >= nt) the others only join the barriers.  Without kIdle
@@ -95,10 +120,17 @@ tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
                 if (o + 1 < ph.op_end) nxt = ops[o + 1].h;
                 if constexpr (NS == 2) {
                     if (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG) {
+                        // every gradient op of a sweep has its own slot and each
+                        // wave runs it once: a plain store, no read
                         double v = exec_grad_op<kMaxDenseQubitsAdj>(h, ops + o, a, coef, goff, off, base);
-                        const int width = nt < 64 ? int(nt) : 64;
-                        for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
-                        if ((tid & 63u) == 0) gacc[h.grad * nwaves + (tid >> 6)] += v;
+                        if (nt >= 64) {
+                            v = wave_sum_lane63(v);
+                            if ((tid & 63u) == 63u) gacc[h.grad * nwaves + (tid >> 6)] = v;
+                        } else {
+                            const int width = int(nt);
+                            for (int d = width >> 1; d > 0; d >>= 1) v += __shfl_xor(v, d, width);
+                            if (tid == 0) gacc[h.grad * nwaves] = v;
+                        }
                         continue;
                     }
                 }
