@@ -724,25 +724,32 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
 }
 
 // <psi|H|psi>, optionally writing lam = H psi.
-// lam = H psi and the block partials of <psi|lam> in scratch[2] (grid of
-// them, returned); groups / pt hold the uploaded tables and must outlive the
-// stream's copies (the caller synchronises).
-svmi_status pauli_launch(svmi_handle h, const c64* psi, c64* lam, const svmi_pauli_term* terms, size_t n_terms,
-                         std::vector<PauliGroup>& groups, std::vector<PauliTerm>& pt, uint32_t* grid_out)
+// The Pauli tables of `terms` into scratch[0..1] (groups / pt hold them on
+// the host and must outlive the stream's copies: the caller synchronises).
+svmi_status pauli_upload(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms, std::vector<PauliGroup>& groups,
+                         std::vector<PauliTerm>& pt)
 {
     group_terms(terms, n_terms, groups, pt);
-    const uint32_t grid = pauli_grid(h->dim);
     SVMI_TRY(h->scratch[0].ensure(std::max<size_t>(groups.size(), 1) * sizeof(PauliGroup)));
     SVMI_TRY(h->scratch[1].ensure(std::max<size_t>(pt.size(), 1) * sizeof(PauliTerm)));
-    SVMI_TRY(h->scratch[2].ensure(size_t(grid) * sizeof(double)));
     if (!groups.empty()) {
         SVMI_HIP(hipMemcpyAsync(h->scratch[0].p, groups.data(), groups.size() * sizeof(PauliGroup),
                                 hipMemcpyHostToDevice, h->stream));
         SVMI_HIP(hipMemcpyAsync(h->scratch[1].p, pt.data(), pt.size() * sizeof(PauliTerm), hipMemcpyHostToDevice,
                                 h->stream));
     }
-    SVMI_HIP(launch_pauli(h->stream, psi, lam, h->dim, h->scratch[0].as<PauliGroup>(), uint32_t(groups.size()),
-                          h->scratch[1].as<PauliTerm>(), h->scratch[2].as<double>()));
+    return SVMI_OK;
+}
+
+// lam = H psi (and copy = psi if given) with the uploaded tables of n_groups
+// groups; the block partials of <psi|lam> in scratch[2] (grid of them,
+// returned).
+svmi_status pauli_launch(svmi_handle h, const c64* psi, c64* lam, c64* copy, size_t n_groups, uint32_t* grid_out)
+{
+    const uint32_t grid = pauli_grid(h->dim);
+    SVMI_TRY(h->scratch[2].ensure(size_t(grid) * sizeof(double)));
+    SVMI_HIP(launch_pauli(h->stream, psi, lam, h->dim, h->scratch[0].as<PauliGroup>(), uint32_t(n_groups),
+                          h->scratch[1].as<PauliTerm>(), h->scratch[2].as<double>(), copy));
     *grid_out = grid;
     return SVMI_OK;
 }
@@ -753,7 +760,8 @@ svmi_status pauli_device(svmi_handle h, const c64* psi, c64* lam, const svmi_pau
     std::vector<PauliGroup> groups;
     std::vector<PauliTerm> pt;
     uint32_t grid = 0;
-    SVMI_TRY(pauli_launch(h, psi, lam, terms, n_terms, groups, pt, &grid));
+    SVMI_TRY(pauli_upload(h, terms, n_terms, groups, pt));
+    SVMI_TRY(pauli_launch(h, psi, lam, nullptr, groups.size(), &grid));
     std::vector<double> part(grid);
     SVMI_HIP(hipMemcpyAsync(part.data(), h->scratch[2].p, grid * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     SVMI_HIP(hipStreamSynchronize(h->stream));
@@ -1154,16 +1162,17 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
     // synchronisation; one copy-back and one sync at the end
     SVMI_TRY(h->aux[0].ensure(h->dim * sizeof(c64)));
     SVMI_TRY(h->aux[1].ensure(h->dim * sizeof(c64)));
-    // forward
-    SVMI_TRY(exec_program(h, h->prog, h->dprog, h->sv, nullptr, nullptr, nullptr, &h->jit));
-    // phi = psi, lam = H psi, partials of E = <psi|lam>
-    c64* phi = h->aux[0].as<c64>();
-    c64* lam = h->aux[1].as<c64>();
-    SVMI_HIP(launch_copy(h->stream, phi, h->sv, h->dim));
+    // the observable's tables go up while the stream is idle
     std::vector<PauliGroup> groups;
     std::vector<PauliTerm> pt;
+    SVMI_TRY(pauli_upload(h, terms, n_terms, groups, pt));
+    // forward
+    SVMI_TRY(exec_program(h, h->prog, h->dprog, h->sv, nullptr, nullptr, nullptr, &h->jit));
+    // phi = psi and lam = H psi in one pass, partials of E = <psi|lam>
+    c64* phi = h->aux[0].as<c64>();
+    c64* lam = h->aux[1].as<c64>();
     uint32_t grid = 0;
-    SVMI_TRY(pauli_launch(h, h->sv, lam, terms, n_terms, groups, pt, &grid));
+    SVMI_TRY(pauli_launch(h, h->sv, lam, phi, groups.size(), &grid));
     for (size_t p = 0; p < n_params; p++) grad_out[p] = 0.0;
     // backward (planned once per circuit and parameter map)
     const size_t n_slots = 3 * h->gates.size();

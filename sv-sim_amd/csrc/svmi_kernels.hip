@@ -183,13 +183,12 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     const uint64_t base = tile_base(sw, tile);
 
     if (active) {
-        c64 r[kRegs];
+        // both states' loads are issued before either waits
+        c64 r[kRegs], r1[NS == 2 ? kRegs : 1];
         tile_fetch(r, s0, sw, ht, tile, tid, nt, init_zero);
+        if constexpr (NS == 2) tile_fetch(r1, s1, sw, ht, tile, tid, nt, false);
         tile_put(r, t0, tid, nt);
-        if (NS == 2) {
-            tile_fetch(r, s1, sw, ht, tile, tid, nt, false);
-            tile_put(r, t1, tid, nt);
-        }
+        if constexpr (NS == 2) tile_put(r1, t1, tid, nt);
     }
     if (NS == 2)
         for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
@@ -975,7 +974,7 @@ hipError_t launch_sample(hipStream_t st, const c64* s, uint32_t nthr, uint32_t c
 __global__ void __launch_bounds__(256)
 k_pauli(const c64* __restrict__ psi, c64* __restrict__ lam, uint64_t dim,
         const PauliGroup* __restrict__ groups, uint32_t ngroups, const PauliTerm* __restrict__ terms,
-        double* __restrict__ partials)
+        double* __restrict__ partials, c64* __restrict__ copy)
 {
     __shared__ double red[256];
     double acc = 0.0;
@@ -996,6 +995,7 @@ k_pauli(const c64* __restrict__ psi, c64* __restrict__ lam, uint64_t dim,
         }
         if (lam) lam[y] = l;
         const c64 p = psi[y];
+        if (copy) copy[y] = p;
         acc += p.re * l.re + p.im * l.im;
     }
     red[threadIdx.x] = acc;
@@ -1010,10 +1010,10 @@ k_pauli(const c64* __restrict__ psi, c64* __restrict__ lam, uint64_t dim,
 uint32_t pauli_grid(uint64_t dim) { return grid_for(dim, 256) > 2048 ? 2048 : grid_for(dim, 256); }
 
 hipError_t launch_pauli(hipStream_t st, const c64* psi, c64* lam, uint64_t dim, const PauliGroup* groups,
-                        uint32_t ngroups, const PauliTerm* terms, double* partials)
+                        uint32_t ngroups, const PauliTerm* terms, double* partials, c64* copy)
 {
     hipLaunchKernelGGL(k_pauli, dim3(pauli_grid(dim)), dim3(256), 0, st, psi, lam, dim, groups, ngroups,
-                       terms, partials);
+                       terms, partials, copy);
     return hipGetLastError();
 }
 

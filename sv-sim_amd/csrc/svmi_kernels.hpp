@@ -93,8 +93,9 @@ hipError_t launch_sample(hipStream_t st, const c64* s, uint32_t nthr, uint32_t c
                          const uint64_t* jobs, uint32_t njobs, const double* offsets,
                          const double* draws, uint64_t* out);
 uint32_t pauli_grid(uint64_t dim);
+// copy (optional): also copy[y] = psi[y]
 hipError_t launch_pauli(hipStream_t st, const c64* psi, c64* lam, uint64_t dim, const PauliGroup* groups,
-                        uint32_t ngroups, const PauliTerm* terms, double* partials);
+                        uint32_t ngroups, const PauliTerm* terms, double* partials, c64* copy = nullptr);
 // dst[seg.dst] = sum of src[seg.src, seg.src + seg.len), per segment, in a
 // fixed order (one workgroup per segment).
 struct SumSegment {
