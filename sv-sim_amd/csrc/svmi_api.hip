@@ -338,8 +338,13 @@ svmi_status verify_batch(const BatchPlan& bp)
                 if (ph.op_end > bp.ops.size() || ph.op_begin > ph.op_end || ph.op_begin < sw.op_begin ||
                     ph.op_end > sw.op_end)
                     return fail(SVMI_ERR_STATE, "batch plan: bad phase");
-                for (int r = 0; r < kRegBits; r++)
-                    if (ph.r[r] >= sw.k) return fail(SVMI_ERR_STATE, "batch plan: bad register bit");
+                // adjoint phases: 3 tile bits, then k (lambda's half of the LDS)
+                const bool two = b.kind == BSTEP_ADJ;
+                for (int r = 0; r < kRegBits; r++) {
+                    const bool state_bit = two && r == kRegBits - 1;
+                    if (state_bit ? ph.r[r] != sw.k : ph.r[r] >= sw.k)
+                        return fail(SVMI_ERR_STATE, "batch plan: bad register bit");
+                }
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++)
                     if (bp.ops[o].h.coef > bp.coef.size()) return fail(SVMI_ERR_STATE, "batch plan: bad coef offset");
             }
@@ -1410,9 +1415,12 @@ void host_standalone(std::vector<c64>& s, int n, const StandaloneOp& so, const s
 void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, const c64* coef, const uint64_t* hitab,
                int ns, c64* s0, c64* s1, uint64_t tile, bool init_zero, double* gsum_tile)
 {
-    const uint32_t tsize = 1u << sw.k, ngroups = tsize >> kRegBits;
+    // as the kernels: one LDS-like array, lambda's half at [2^k, 2^(k+1));
+    // a group is 16 amplitudes (two states: 8 of phi then 8 of lambda, the
+    // phase's last register bit being k)
+    const uint32_t tsize = 1u << sw.k, ngroups = sweep_threads(sw.k, ns);
     const uint32_t cmask = (1u << sw.c) - 1u;
-    std::vector<c64> t0(tsize), t1(tsize);
+    std::vector<c64> t(size_t(tsize) * ns);
     std::vector<uint64_t> gidx(tsize);
     for (uint32_t i = 0; i < sw.n_grad; i++) gsum_tile[i] = 0.0;
     const uint64_t base = tile_base(sw, tile);
@@ -1421,8 +1429,8 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
         if (ops[o].h.type == DOP_PHASESET && ops[o].h.s1) xf[ops[o].h.grad] = phase_factor(ops[o].h, coef, base);
     for (uint32_t l = 0; l < tsize; l++) {
         gidx[l] = base | hitab[sw.hitab + (l >> sw.c)] | (l & cmask);
-        t0[l] = init_zero ? c64{gidx[l] == 0 ? 1.0 : 0.0, 0.0} : s0[gidx[l]];
-        if (ns == 2) t1[l] = s1[gidx[l]];
+        t[l] = init_zero ? c64{gidx[l] == 0 ? 1.0 : 0.0, 0.0} : s0[gidx[l]];
+        if (ns == 2) t[tsize + l] = s1[gidx[l]];
     }
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
         const PhaseInfo& ph = phases[p];
@@ -1430,30 +1438,21 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
         phase_offsets(ph, off);
         for (uint32_t g = 0; g < ngroups; g++) {
             const uint32_t goff = group_offset(ph, g);
-            // as the kernels: one state's group, or phi's then lambda's
-            c64 a[kRegs], ab[2 * kRegs];
-            for (int r = 0; r < kRegs; r++) {
-                a[r] = ab[r] = t0[goff | off[r]];
-                ab[kRegs + r] = ns == 2 ? t1[goff | off[r]] : c64{0.0, 0.0};
-            }
+            c64 a[kRegs];
+            for (int r = 0; r < kRegs; r++) a[r] = t[goff | off[r]];
             for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                 const DOp& op = ops[o];
                 if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG)
-                    gsum_tile[op.h.grad] += exec_grad_op(op.h, &op, ab, coef, goff, off, base);
-                else if (ns == 2)
-                    exec_state_op<kRegBits, 2 * kRegs>(op.h, &op, ab, coef, goff, off, base, xf.data());
+                    gsum_tile[op.h.grad] += exec_grad_op<kRegBits, kRegsAdj>(op.h, &op, a, coef, goff, off, base);
                 else
                     exec_state_op(op.h, &op, a, coef, goff, off, base, xf.data());
             }
-            for (int r = 0; r < kRegs; r++) {
-                t0[goff | off[r]] = ns == 2 ? ab[r] : a[r];
-                if (ns == 2) t1[goff | off[r]] = ab[kRegs + r];
-            }
+            for (int r = 0; r < kRegs; r++) t[goff | off[r]] = a[r];
         }
     }
     for (uint32_t l = 0; l < tsize; l++) {
-        s0[gidx[l]] = t0[l];
-        if (ns == 2) s1[gidx[l]] = t1[l];
+        s0[gidx[l]] = t[l];
+        if (ns == 2) s1[gidx[l]] = t[tsize + l];
     }
 }
 

@@ -41,12 +41,13 @@ SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[N], const c64* __restrict
     }
 }
 
-template <int kMaxQ>
-SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&pl)[2 * kRegs], const c64* __restrict__ M)
+template <int kMaxQ, int H>
+SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&pl)[2 * H], const c64* __restrict__ M)
 {
     double sum = 0.0;
     switch (smask) {
-#define SVMI_GRAD_CASE(S) SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ) sum = grad_dense<S>(pl, M))
+#define SVMI_GRAD_CASE(S) \
+    SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ && (S & ~(H - 1)) == 0) sum = (grad_dense<S, H>(pl, M)))
         SVMI_GRAD_CASE(1) SVMI_GRAD_CASE(2) SVMI_GRAD_CASE(3) SVMI_GRAD_CASE(4)
         SVMI_GRAD_CASE(5) SVMI_GRAD_CASE(6) SVMI_GRAD_CASE(7) SVMI_GRAD_CASE(8)
         SVMI_GRAD_CASE(9) SVMI_GRAD_CASE(10) SVMI_GRAD_CASE(11) SVMI_GRAD_CASE(12)
@@ -253,22 +254,22 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
     }
 }
 
-// Gradient op: Re sum conj(lam) K phi over this group.
-template <int kMaxQ = kRegBits>
-// pl = phi's register group then lambda's.
-SVMI_HD inline double exec_grad_op(const DOpHdr& h, const DOp* __restrict__ op, const c64 (&pl)[2 * kRegs],
+// Gradient op: Re sum conj(lam) K phi over this group;
+// pl = phi's H amplitudes of the group then lambda's.
+template <int kMaxQ, int H>
+SVMI_HD inline double exec_grad_op(const DOpHdr& h, const DOp* __restrict__ op, const c64 (&pl)[2 * H],
                                    const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
                                    uint64_t base)
 {
-    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense<kMaxQ>(h.smask, pl, coef + h.coef);
+    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense<kMaxQ, H>(h.smask, pl, coef + h.coef);
     // DOP_GRAD_DIAG
     const uint32_t e = diag_ext_index(*op, base);
     const c64* tab = coef + h.coef;
     double s = 0.0;
 #pragma unroll
-    for (int r = 0; r < kRegs; r++) {
+    for (int r = 0; r < H; r++) {
         const c64 v = cmul(tab[diag_index(*op, goff | off[r], e)], pl[r]);
-        s += pl[kRegs + r].re * v.re + pl[kRegs + r].im * v.im;
+        s += pl[H + r].re * v.re + pl[H + r].im * v.im;
     }
     return s;
 }

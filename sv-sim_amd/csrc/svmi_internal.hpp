@@ -65,6 +65,15 @@ constexpr int kRegBits = 4;                 // register bits per phase
 constexpr int kRegs = 1 << kRegBits;        // amplitudes per thread per group
 constexpr int kMaxTileQubits = 13;          // LDS tile: 2^13 * 16 B = 128 KiB
 constexpr int kMaxTileQubitsAdj = 11;       // two states (phi, lambda): 2 * 32 KiB
+// Two-state phases have 3 register bits; their fourth register "bit" is k,
+// the bit above the tile, which selects lambda's half of the LDS (phi at
+// [0, 2^k), lambda at [2^k, 2^(k+1))).  A thread's 16-amplitude group is then
+// phi's 8 amplitudes then lambda's, every op acts on both, and a tile has
+// 2^(k-3) threads -- the register footprint of a one-state group.
+constexpr int kRegBitsAdj = 3;
+constexpr int kRegsAdj = 1 << kRegBitsAdj;   // amplitudes per state per thread
+// threads of a tile of 2^k amplitudes with ns states
+SVMI_HD inline uint32_t sweep_threads(uint32_t k, int ns) { return 1u << (k - (ns == 2 ? kRegBitsAdj : kRegBits)); }
 constexpr int kMaxDenseQubitsAdj = 2;       // two-state sweeps run dense ops of <= 2 qubits
                                             // (wider gates: full-state passes on both states)
 constexpr int kThreads = 256;               // threads of the reduction / Pauli workgroups
@@ -230,22 +239,22 @@ SVMI_HD inline void apply_dense(c64 (&a)[N], const c64* __restrict__ M)
 }
 
 // Re sum_groups conj(l) . (M p) over the slots of S (state untouched);
-// pl = phi's group then lambda's.
-template <int S>
-SVMI_HD inline double grad_dense(const c64 (&pl)[2 * kRegs], const c64* __restrict__ M)
+// pl = phi's H amplitudes then lambda's (S within the low log2(H) slots).
+template <int S, int H>
+SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__ M)
 {
     constexpr int q = SlotMask<S>::q;
     constexpr int D = 1 << q;
     double sum = 0.0;
 #pragma unroll
-    for (int o = 0; o < (kRegs >> q); o++) {
+    for (int o = 0; o < (H >> q); o++) {
 #pragma unroll
         for (int i = 0; i < D; i++) {
             c64 acc = c64{0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < D; j++)
                 acc = cmac(acc, M[i * D + j], pl[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)]);
-            const c64 lv = pl[kRegs + (SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i))];
+            const c64 lv = pl[H + (SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i))];
             sum += lv.re * acc.re + lv.im * acc.im;
         }
     }

@@ -20,42 +20,47 @@ namespace svmi {
 // ------------------------------------------------------------------ sweeps
 
 // Tile staging.  A tile is 2^k amplitudes; thread tid owns tile-local
-// indices l = tid + nt*j (j < 16); global index = base | row(l >> c) | (l & cmask).
-// LDS holds the tile XOR-swizzled (lds_slot).
+// indices l = tid + nt*j (j < J: 16 for one state, 8 per state for two);
+// global index = base | row(l >> c) | (l & cmask).  LDS holds the tile
+// XOR-swizzled (lds_slot) at LDS index lo + l: lo = 0, or 2^k for lambda's
+// half of a two-state tile.
 
 // HBM (or |0..0> when init_zero) -> registers r[j] for tile `tile`.
+template <int J>
 __device__ __attribute__((always_inline)) inline void
-tile_fetch(c64 (&r)[kRegs], const c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht,
+tile_fetch(c64 (&r)[J], const c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht,
            uint64_t tile, uint32_t tid, uint32_t nt, bool init_zero)
 {
     const uint64_t base = tile_base(sw, tile);
     const uint32_t cmask = (1u << sw.c) - 1u;
 #pragma unroll
-    for (int j = 0; j < kRegs; j++) {
+    for (int j = 0; j < J; j++) {
         const uint32_t l = tid + nt * j;
         const uint64_t g = base | ht[l >> sw.c] | (l & cmask);
         r[j] = init_zero ? c64{g == 0 ? 1.0 : 0.0, 0.0} : s[g];
     }
 }
 
+template <int J>
 __device__ __attribute__((always_inline)) inline void
-tile_put(const c64 (&r)[kRegs], c64* t, uint32_t tid, uint32_t nt)
+tile_put(const c64 (&r)[J], c64* t, uint32_t lo, uint32_t tid, uint32_t nt)
 {
 #pragma unroll
-    for (int j = 0; j < kRegs; j++) t[lds_slot(tid + nt * j)] = r[j];
+    for (int j = 0; j < J; j++) t[lds_slot(lo + tid + nt * j)] = r[j];
 }
 
 // LDS -> HBM for tile `tile`.
+template <int J>
 __device__ __attribute__((always_inline)) inline void
-tile_store(const c64* t, c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht, uint64_t tile,
-           uint32_t tid, uint32_t nt)
+tile_store(const c64* t, uint32_t lo, c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht,
+           uint64_t tile, uint32_t tid, uint32_t nt)
 {
     const uint64_t base = tile_base(sw, tile);
     const uint32_t cmask = (1u << sw.c) - 1u;
 #pragma unroll
-    for (int j = 0; j < kRegs; j++) {
+    for (int j = 0; j < J; j++) {
         const uint32_t l = tid + nt * j;
-        s[base | ht[l >> sw.c] | (l & cmask)] = t[lds_slot(l)];
+        s[base | ht[l >> sw.c] | (l & cmask)] = t[lds_slot(lo + l)];
     }
 }
 
@@ -84,14 +89,14 @@ __device__ __attribute__((always_inline)) inline double wave_sum_lane63(double v
     return v;
 }
 
-// The phases of one sweep on the tile in LDS (t0, and t1 for NS = 2).
+// The phases of one sweep on the tile in LDS (t0; for NS = 2 phi then lambda).
 // Threads tid < nt own one register group per phase; with kIdle (batched
 // kernel: blockDim >= nt) the others only join the barriers.  Without kIdle
 // every thread is active, which keeps the phase body free of divergence
 // (and its register groups out of scratch).  Ends with a barrier.
 template <int NS, bool kIdle>
 __device__ __attribute__((always_inline)) inline void
-tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
+tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
             const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops, const c64* __restrict__ coef,
             uint64_t base, uint32_t tid, uint32_t nt)
 {
@@ -103,14 +108,12 @@ tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
             uint32_t off[kRegs];
             phase_offsets(ph, off);
             const uint32_t goff = group_offset(ph, tid);
-            // NS = 2: phi's group in a[0..15], lambda's in a[16..31]; each
-            // op is dispatched once for both
-            c64 a[NS * kRegs];
+            // NS = 2: the phase's last register bit is k, so a[0..7] are
+            // phi's amplitudes and a[8..15] lambda's (t1 = t0 + 2^k) and every
+            // op, dispatched once, acts on both
+            c64 a[kRegs];
 #pragma unroll
-            for (int r = 0; r < kRegs; r++) {
-                a[r] = t0[lds_slot(goff | off[r])];
-                if constexpr (NS == 2) a[kRegs + r] = t1[lds_slot(goff | off[r])];
-            }
+            for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(goff | off[r])];
             // op headers are fetched one op ahead so the scalar-load latency
             // of the next op overlaps the arithmetic of the current one
             DOpHdr nxt;
@@ -122,7 +125,7 @@ tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
                     if (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG) {
                         // every gradient op of a sweep has its own slot and each
                         // wave runs it once: a plain store, no read
-                        double v = exec_grad_op<kMaxDenseQubitsAdj>(h, ops + o, a, coef, goff, off, base);
+                        double v = exec_grad_op<kMaxDenseQubitsAdj, kRegsAdj>(h, ops + o, a, coef, goff, off, base);
                         if (nt >= 64) {
                             v = wave_sum_lane63(v);
                             if ((tid & 63u) == 63u) gacc[h.grad * nwaves + (tid >> 6)] = v;
@@ -134,14 +137,10 @@ tile_phases(c64* t0, c64* t1, double* gacc, const c64* xf, const SweepInfo& sw,
                         continue;
                     }
                 }
-                exec_state_op<NS == 2 ? kMaxDenseQubitsAdj : kRegBits, NS * kRegs>(h, ops + o, a, coef, goff, off,
-                                                                                  base, xf);
+                exec_state_op<NS == 2 ? kMaxDenseQubitsAdj : kRegBits>(h, ops + o, a, coef, goff, off, base, xf);
             }
 #pragma unroll
-            for (int r = 0; r < kRegs; r++) {
-                t0[lds_slot(goff | off[r])] = a[r];
-                if constexpr (NS == 2) t1[lds_slot(goff | off[r])] = a[kRegs + r];
-            }
+            for (int r = 0; r < kRegs; r++) t0[lds_slot(goff | off[r])] = a[r];
         }
         __syncthreads();
     }
@@ -160,7 +159,8 @@ tile_factors(c64* xf, const SweepInfo& sw, const DOp* __restrict__ ops, const c6
     }
 }
 
-// LDS of a tile: t0 [t1] | phase factors | gradient accumulators.
+// LDS of a tile: t0 (NS = 2: phi's 2^k then lambda's) | phase factors |
+// gradient accumulators.
 // One tile of one sweep: load, phases, store (and for NS = 2 the tile's
 // gradient partials).  init_zero: the tile is generated as |0..0>.
 template <int NS, bool kIdle>
@@ -173,7 +173,6 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
 {
     const uint32_t tsize = 1u << sw.k;
     c64* t0 = reinterpret_cast<c64*>(smem);
-    c64* t1 = t0 + (NS == 2 ? tsize : 0);
     c64* xf = t0 + tsize * NS;
     double* gacc = reinterpret_cast<double*>(xf + sw.n_xf);
     const uint32_t tid = threadIdx.x;
@@ -182,22 +181,23 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     const uint64_t* ht = hitab + sw.hitab;
     const uint64_t base = tile_base(sw, tile);
 
+    constexpr int J = NS == 2 ? kRegsAdj : kRegs;   // amplitudes per thread per state
     if (active) {
         // both states' loads are issued before either waits
-        c64 r[kRegs], r1[NS == 2 ? kRegs : 1];
+        c64 r[J], r1[J];
         tile_fetch(r, s0, sw, ht, tile, tid, nt, init_zero);
         if constexpr (NS == 2) tile_fetch(r1, s1, sw, ht, tile, tid, nt, false);
-        tile_put(r, t0, tid, nt);
-        if constexpr (NS == 2) tile_put(r1, t1, tid, nt);
+        tile_put(r, t0, 0u, tid, nt);
+        if constexpr (NS == 2) tile_put(r1, t0, tsize, tid, nt);
     }
     if (NS == 2)
         for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
     tile_factors(xf, sw, ops, coef, base, tid);
     __syncthreads();
-    tile_phases<NS, kIdle>(t0, t1, gacc, xf, sw, phases, ops, coef, base, tid, nt);
+    tile_phases<NS, kIdle>(t0, gacc, xf, sw, phases, ops, coef, base, tid, nt);
     if (active) {
-        tile_store(t0, s0, sw, ht, tile, tid, nt);
-        if (NS == 2) tile_store(t1, s1, sw, ht, tile, tid, nt);
+        tile_store<J>(t0, 0u, s0, sw, ht, tile, tid, nt);
+        if constexpr (NS == 2) tile_store<J>(t0, tsize, s1, sw, ht, tile, tid, nt);
     }
     // gradient partials are stored grad-major ([i][tile]) so the sum over
     // tiles of one gradient reads one contiguous segment
@@ -250,14 +250,14 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
     tile_fetch(nx, s0, sw, ht, tile_offset + t, tid, nt, false);
     for (;;) {
         const uint64_t base = tile_base(sw, tile_offset + t);
-        tile_put(nx, t0, tid, nt);
+        tile_put(nx, t0, 0u, tid, nt);
         tile_factors(xf, sw, ops, coef, base, tid);
         __syncthreads();
         const uint64_t tn = t + gridDim.x;
         const bool more = tn < n_tiles;
         if (more) tile_fetch(nx, s0, sw, ht, tile_offset + tn, tid, nt, false);
-        tile_phases<1, false>(t0, nullptr, nullptr, xf, sw, phases, ops, coef, base, tid, nt);
-        tile_store(t0, s0, sw, ht, tile_offset + t, tid, nt);
+        tile_phases<1, false>(t0, nullptr, xf, sw, phases, ops, coef, base, tid, nt);
+        tile_store<kRegs>(t0, 0u, s0, sw, ht, tile_offset + t, tid, nt);
         if (!more) break;
         __syncthreads();  // the store's LDS reads finish before the next put
         t = tn;
@@ -378,7 +378,7 @@ size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad, uint32_t 
 {
     if (kind == BSTEP_FWD) return (size_t(16) << k) + size_t(16) * max_xf;
     if (kind == BSTEP_ADJ) {
-        const size_t nwaves = ((size_t(1) << (k - kRegBits)) + 63) / 64;
+        const size_t nwaves = (size_t(sweep_threads(k, 2)) + 63) / 64;
         return (size_t(32) << k) + size_t(16) * max_xf + size_t(8) * max_grad * nwaves;
     }
     return 0;
@@ -388,7 +388,9 @@ hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L
 {
     const size_t smem = batch_launch_smem(L.kind, L.k, L.max_grad, L.max_xf);
     if (smem > g_max_smem) return hipErrorInvalidConfiguration;
-    const uint32_t threads = (L.kind == BSTEP_FWD || L.kind == BSTEP_ADJ) ? 1u << (L.k - kRegBits) : kThreads;
+    const uint32_t threads = L.kind == BSTEP_FWD ? sweep_threads(L.k, 1)
+                             : L.kind == BSTEP_ADJ ? sweep_threads(L.k, 2)
+                                                   : kThreads;
     const uint64_t max_grid = 1ull << 30;
     for (uint64_t t0 = 0; t0 < L.n_tasks; t0 += max_grid) {
         const uint32_t cnt = uint32_t(L.n_tasks - t0 < max_grid ? L.n_tasks - t0 : max_grid);
@@ -418,7 +420,7 @@ hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L
 
 size_t sweep_smem_bytes(const SweepInfo& sw, int ns)
 {
-    const size_t nt = size_t(1) << (sw.k - kRegBits);
+    const size_t nt = sweep_threads(sw.k, ns);
     const size_t nwaves = (nt + 63) / 64;
     return (size_t(16) << sw.k) * ns + size_t(16) * sw.n_xf + size_t(8) * sw.n_grad * nwaves;
 }
@@ -446,7 +448,7 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
                         const PhaseInfo* phases, const DOp* ops, const c64* coef,
                         const uint64_t* hitab, double* grad_partials, int pipeline)
 {
-    const uint32_t nt = 1u << (sw.k - kRegBits);
+    const uint32_t nt = sweep_threads(sw.k, ns);
     const size_t smem = sweep_smem_bytes(sw, ns);
     if (smem > g_max_smem) return hipErrorInvalidConfiguration;
     if (ns == 1 && (pipeline == 1 || pipeline == 2)) {

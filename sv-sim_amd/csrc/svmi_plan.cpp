@@ -505,6 +505,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                 const PlanOptions& opt)
 {
     Encoder enc(prog);
+    const int rb = prog.n_states == 2 ? kRegBitsAdj : kRegBits;   // register bits of a phase
     std::deque<LOp> owned;     // merged ops (live until encoded)
     std::deque<SOp> sowned;
     int tpos[64];
@@ -606,7 +607,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         // candidate bits: tile bits of the first pending ops
         uint32_t cand = 0;
         for (size_t i = 0; i < remaining.size() && i < 96; i++) cand |= remaining[i].lm;
-        while (popc(R) < kRegBits) {
+        while (popc(R) < rb) {
             long best = -(1L << 60);
             int best_b = -1;
             for (int b = 0; b < k; b++) {
@@ -623,10 +624,10 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         std::vector<const SOp*> ph;
         std::vector<Pending> rest;
         take_with(R, &ph, &rest);
-        // pad R to 4 bits with the lowest free tile bits above the low qubits first
-        for (int l = 0; l < k && popc(R) < kRegBits; l++)
+        // pad R to rb bits with the lowest free tile bits above the low qubits first
+        for (int l = 0; l < k && popc(R) < rb; l++)
             if (!((R >> l) & 1) && l >= c) R |= 1u << l;
-        for (int l = 0; l < k && popc(R) < kRegBits; l++)
+        for (int l = 0; l < k && popc(R) < rb; l++)
             if (!((R >> l) & 1)) R |= 1u << l;
         PhaseInfo pi;
         std::memset(&pi, 0, sizeof(pi));
@@ -639,7 +640,11 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                 slot_of[l] = ns++;
             }
         int phys_of_slot[kRegBits];
-        for (int sl = 0; sl < kRegBits; sl++) phys_of_slot[sl] = tpos[pi.r[sl]];
+        for (int sl = 0; sl < rb; sl++) phys_of_slot[sl] = tpos[pi.r[sl]];
+        if (rb < kRegBits) {   // two states: the last register bit selects lambda
+            pi.r[kRegBits - 1] = uint32_t(k);
+            phys_of_slot[kRegBits - 1] = -1;
+        }
         std::vector<const SOp*> merged;
         {
             RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged, sowned,
