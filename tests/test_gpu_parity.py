@@ -241,6 +241,50 @@ def test_adjoint_vs_oracle_parameter_shift(n, layers):
     assert np.abs(g - O.param_shift_grad(n, gates, slots, P, terms)).max() < TOL
 
 
+@pytest.mark.parametrize("n,tile", [(3, 12), (5, 12), (9, 6), (13, 12), (14, 8)])
+def test_adjoint_every_parametric_gate_vs_oracle(n, tile):
+    # two-state sweeps: every op kind the backward program can hold (dense 1-
+    # and 2-qubit blocks, diagonals, permutations, phase products, gradient
+    # ops), at tiles from 2 threads (shuffle reduction) to 256 (DPP)
+    rng = np.random.default_rng(700 + n)
+    names = ["U3", "U2", "U1", "RX", "RY", "RZ", "CRX", "CRY", "CRZ", "CU1", "CU3", "RXX", "RZZ", "RYY",
+             "H", "CX", "CZ", "SWAP", "CCX", "T", "S"]
+    gates, slots, p = [], [], 0
+    for _ in range(2):
+        for name in names:
+            k = svmi.ARITY[name]
+            if k > n:
+                continue
+            gates.append((svmi.OP[name], list(rng.choice(n, size=k, replace=False)), *(rng.random(3) * 6 - 3)))
+            slots.append([p, p + 1, p + 2])
+            p += 3
+    slots = np.array(slots)
+    terms = W.ising_terms(n, rng)
+    st = W.random_state(n, seed=n + 40)
+    with svmi.Engine(n) as e:
+        e.set_option(svmi.OPT_TILE_QUBITS, tile)
+        e.set_state(*st)
+        e.upload(gates)
+        E, g = e.adjoint(terms, slots, p)
+    Eh, gh = svmi.emulate_adjoint(n, gates, terms, slots, p, st, tile_qubits=tile)
+    assert abs(E - O.expval(n, *O.run(n, gates, st), terms)) < TOL
+    assert abs(E - Eh) < 1e-12 and np.abs(g - gh).max() < 1e-12
+    # parameter shift is exact for the slots of its list; finite differences
+    # (5-point, h = 1e-5) for the rest
+    G = O.as_gates(gates)
+    fd = np.zeros(p)
+    nm = ("theta", "phi", "lambda")
+    for gi in range(len(G)):
+        for s_ in range(3):
+            vals = []
+            for h in (1e-5, -1e-5, 2e-5, -2e-5):
+                G2 = G.copy()
+                G2[gi][nm[s_]] += h
+                vals.append(O.expval(n, *O.run(n, G2, st), terms))
+            fd[slots[gi, s_]] = (8 * (vals[0] - vals[1]) - (vals[2] - vals[3])) / 12e-5
+    assert np.abs(g - fd).max() < 1e-8
+
+
 def test_adjoint_reuses_and_replans_backward_program():
     # one engine: repeated call, a different parameter map, then new angles
     n = 9
