@@ -176,7 +176,12 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
  *      launch per level runs the tasks of all circuits at that level. */
 typedef struct svmi_batch_item {
     int32_t n_qubits;
-    int32_t reserved;
+    /* planning of this circuit; 0 = the autotuned table for n_qubits
+       (sv-sim_amd/csrc/svmi_tuning.hpp, written by svmi/autotune.py)      */
+    uint8_t fwd_tile;               /* tile qubits of the forward sweeps, 4..13      */
+    uint8_t adj_tile;               /* tile qubits of the backward sweeps, 4..11     */
+    uint8_t op_cost;                /* fusion op cost + 1 (1..33)                    */
+    uint8_t reserved;
     const svmi_gate* gates;
     size_t n_gates;
     const int32_t* param_slots;     /* 3 per gate, -1 = not trainable */

@@ -19,6 +19,7 @@
 #include "svmi_jit.hpp"
 #include "svmi_qasm.hpp"
 #include "svmi_qir.hpp"
+#include "svmi_tuning.hpp"
 
 using namespace svmi;
 
@@ -373,6 +374,10 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
         const svmi_batch_item& it = items[i];
         if (it.n_qubits < 1 || it.n_qubits > 30)
             return fail(SVMI_ERR_ARG, "batch item " + std::to_string(i) + ": n_qubits must be in [1, 30]");
+        if ((it.fwd_tile && (it.fwd_tile < kRegBits || it.fwd_tile > kMaxTileQubits)) ||
+            (it.adj_tile && (it.adj_tile < kRegBits || it.adj_tile > kMaxTileQubitsAdj)) || it.op_cost > 33)
+            return fail(SVMI_ERR_ARG, "batch item " + std::to_string(i) + ": fwd_tile must be in [4, 13], adj_tile in "
+                                                                             "[4, 11], op_cost + 1 in [1, 33]");
         SVMI_TRY(check_gates(it.gates, it.n_gates, it.n_qubits));
         SVMI_TRY(check_terms(it.terms, it.n_terms, it.n_qubits));
         SVMI_TRY(check_params(it.gates, it.n_gates, it.param_slots, it.n_params));
@@ -391,12 +396,18 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
             for (size_t i = next++; i < n_items; i = next++) {
                 const svmi_batch_item& it = items[i];
                 ni[i] = std::max(int(it.n_qubits), kRegBits);
+                // tile sizes and fusion op cost: the item's, else the
+                // autotuned table's for its qubit count
+                const BatchTuning& tu = batch_tuning(ni[i]);
                 PlanOptions of;
+                of.tile_qubits = it.fwd_tile ? it.fwd_tile : tu.fwd_tile;
+                of.op_overhead = it.op_cost ? it.op_cost - 1 : tu.op_cost;
                 plan_forward(it.gates, it.n_gates, ni[i], of, bp.fwd[i]);
                 std::vector<AdjEntry> entries;
                 adjoint_entries(it.gates, it.n_gates, it.param_slots, entries);
                 PlanOptions oa;
-                oa.tile_qubits = kMaxTileQubitsAdj;
+                oa.tile_qubits = it.adj_tile ? it.adj_tile : tu.adj_tile;
+                oa.op_overhead = of.op_overhead;
                 build_adjoint_program(entries, ni[i], oa, bp.adj[i]);
                 group_terms(it.terms, it.n_terms, pg[i], pt[i]);
             }
@@ -410,8 +421,10 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
         for (const Program* pr : {&bp.fwd[i], &bp.adj[i]})
             for (const Step& stp : pr->steps)
                 if (stp.kind != Step::SWEEP)
-                    return fail(SVMI_ERR_UNSUPPORTED, "batch item " + std::to_string(i) +
-                                                          ": 5-qubit gates (C4X) are not supported in batches");
+                    return fail(SVMI_ERR_UNSUPPORTED,
+                                "batch item " + std::to_string(i) +
+                                    ": an op does not fit a tile (a 5-qubit gate such as C4X, or a tile too small "
+                                    "for the 4 low qubits plus the op's qubits)");
     auto append_program = [&](const Program& pr) {
         const uint32_t op_base = uint32_t(bp.ops.size()), coef_base = uint32_t(bp.coef.size());
         const uint32_t ph_base = uint32_t(bp.phases.size()), ht_base = uint32_t(bp.hitab.size());

@@ -70,7 +70,8 @@ class Stats(ctypes.Structure):
 
 
 class BatchItem(ctypes.Structure):
-    _fields_ = [("n_qubits", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    _fields_ = [("n_qubits", ctypes.c_int32), ("fwd_tile", ctypes.c_uint8), ("adj_tile", ctypes.c_uint8),
+                ("op_cost", ctypes.c_uint8), ("reserved", ctypes.c_uint8),
                 ("gates", ctypes.c_void_p), ("n_gates", ctypes.c_size_t),
                 ("param_slots", ctypes.c_void_p), ("n_params", ctypes.c_size_t),
                 ("terms", ctypes.c_void_p), ("n_terms", ctypes.c_size_t)]
@@ -337,12 +338,17 @@ def _batch_items(items):
         arr[i].n_params = int(it["n_params"])
         arr[i].terms = t.ctypes.data if t.size else None
         arr[i].n_terms = len(t)
+        # planning overrides (0 = the autotuned table)
+        arr[i].fwd_tile = int(it.get("fwd_tile", 0))
+        arr[i].adj_tile = int(it.get("adj_tile", 0))
+        arr[i].op_cost = int(it["op_cost"]) + 1 if it.get("op_cost") is not None else 0
         total += int(it["n_params"])
     return arr, keep, total
 
 
 def batch_adjoint(items, device=0, _emulate=False):
-    """items: list of dicts {n, gates, param_slots, n_params, terms}.
+    """items: list of dicts {n, gates, param_slots, n_params, terms} and,
+    optionally, fwd_tile / adj_tile / op_cost (else the autotuned table).
     Returns (expvals, list of gradient arrays).  One launch per dependency
     level runs every circuit's forward pass, <H> and adjoint sweep."""
     arr, keep, total = _batch_items(items)

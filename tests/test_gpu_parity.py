@@ -354,6 +354,18 @@ def test_batch_matches_single():
         assert np.abs(g - O.param_shift_grad(n, it["gates"], it["param_slots"], it["n_params"], it["terms"])).max() < TOL
 
 
+def test_batch_with_planning_overrides():
+    # per-item tiles and fusion op cost (instead of the autotuned table)
+    items = W.batch_ising(count=5, nmin=5, nmax=13, lmin=1, lmax=3, seed=33)
+    for ov in ({"fwd_tile": 6, "adj_tile": 6, "op_cost": 0}, {"fwd_tile": 13, "adj_tile": 9, "op_cost": 8}):
+        ev, grads = svmi.batch_adjoint([dict(it, **ov) for it in items])
+        for it, e, g in zip(items, ev, grads):
+            n = it["n"]
+            assert abs(e - O.expval(n, *O.run(n, it["gates"]), it["terms"])) < TOL
+            want = O.param_shift_grad(n, it["gates"], it["param_slots"], it["n_params"], it["terms"])
+            assert np.abs(g - want).max() < TOL, ov
+
+
 def test_errors_are_reported():
     with svmi.Engine(3) as e:
         with pytest.raises(svmi.SvmiError) as ex:

@@ -5,6 +5,8 @@ device program on the host with the same op code the HIP kernels use
 (sv-sim_amd/csrc/svmi_exec.hpp), so these tests pin the planner, the gate
 lowering and the adjoint recurrences against the oracle without a GPU.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -167,3 +169,22 @@ def test_specialised_sweep_source_compiles_without_a_device():
     assert "svmi_jit_0" in src and "#error" not in src
     src13, _, _ = svmi.jit_source(14, gates, tile_qubits=13, compile=False)
     assert "__global__" not in src13
+
+
+def test_batch_tuning_table_and_overrides():
+    # the autotuned table (sv-sim_amd/csrc/svmi_tuning.hpp) covers 4..30 qubits
+    # with valid settings; per-item overrides plan differently but give the
+    # same expectations and gradients; out-of-range overrides are errors
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "sv-sim_amd", "csrc", "svmi_tuning.hpp")).read()
+    rows = [tuple(map(int, m)) for m in re.findall(r"\{(\d+), (\d+), (\d+), (\d+)\}", src)]
+    assert [r[0] for r in rows] == list(range(4, 31))
+    for n, fwd, adj, cost in rows:
+        assert 4 <= fwd <= 13 and 4 <= adj <= 11 and 0 <= cost <= 32, (n, fwd, adj, cost)
+    items = W.batch_ising(count=4, nmin=6, nmax=11, lmin=1, lmax=3, seed=21)
+    e0, g0 = svmi.batch_adjoint(items, _emulate=True)
+    e1, g1 = svmi.batch_adjoint([dict(it, fwd_tile=6, adj_tile=7, op_cost=8) for it in items], _emulate=True)
+    assert np.abs(e0 - e1).max() < 1e-12 and max(np.abs(a - b).max() for a, b in zip(g0, g1)) < 1e-12
+    with pytest.raises(svmi.SvmiError) as e:
+        svmi.batch_adjoint([dict(items[0], adj_tile=12)], _emulate=True)
+    assert "adj_tile" in str(e.value)
