@@ -1092,7 +1092,10 @@ svmi_status svmi_measure(svmi_handle h, const double* uniforms, size_t shots, ui
 {
     SVMI_TRY(use_device(h));
     if (shots && (!uniforms || !out)) return fail(SVMI_ERR_ARG, "null draws/output");
-    const int s = std::min(h->n_int, 14);
+    // blocks of 2^12 amplitudes: the staged probabilities of one (35 KiB of
+    // LDS) leave room for four per CU, so loads overlap the scans
+    // (2^14-amplitude blocks fit one per CU: 2.1 TB/s at 30 qubits)
+    const int s = std::min(h->n_int, 12);
     const uint64_t bsize = 1ull << s;
     const uint32_t nthr = uint32_t(std::min<uint64_t>(256, bsize));
     const uint32_t chunk = uint32_t(bsize / nthr);
