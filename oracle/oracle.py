@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libsvsim_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libsvsim_ref.so")
 REF_QIR_SO = os.path.join(HERE, "_ref", "libsvsim_qir_ref.so")
+REF_DM_SO = os.path.join(HERE, "_ref", "libsvsim_dm_ref.so")
 
 # Gate record layout shared by the oracle, the reference driver and svmi.h:
 # {int32 op; int32 qb[5]; double theta, phi, lambda;}  -> 48 bytes.
@@ -312,3 +313,66 @@ def ref_qir_run(capacity, n, gates, re, im, nthreads=1):
 
 
 _ref_qir = None
+
+
+# ---- density-matrix engine (svsim_qsharp_noise/svsim_cpu_omp.hpp) ----
+
+def dm_qir_apply(n, gates, state=None, nthreads=0):
+    """The density-matrix engine on n qubits with the oracle: rho starts as
+    |0..0><0..0| (or the given 2^(2n)-entry (re, im), index row | (col << n)).
+    Returns (re, im, probs, outcomes)."""
+    g = as_qir_gates(gates)
+    dim = 1 << (2 * n)
+    if state is None:
+        re = np.zeros(dim)
+        im = np.zeros(dim)
+        re[0] = 1.0
+    else:
+        re = np.array(state[0], dtype=np.float64, copy=True)
+        im = np.array(state[1], dtype=np.float64, copy=True)
+        assert len(re) == dim and len(im) == dim
+    nm = int(np.count_nonzero(g["op"] == QIR["Measure"]))
+    probs = np.zeros(max(nm, 1))
+    outs = np.zeros(max(nm, 1), dtype=np.int32)
+    L = lib()
+    L.orc_dm_qir_apply.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp, ctypes.c_int]
+    L.orc_dm_qir_apply.restype = ctypes.c_int
+    rc = L.orc_dm_qir_apply(n, _ptr(re), _ptr(im), _ptr(g), len(g), _ptr(probs), _ptr(outs), int(nthreads))
+    if rc != 0:
+        raise ValueError("oracle rejected the QIR gate list (bad op, target or swap qubit)")
+    return re, im, probs[:nm], outs[:nm]
+
+
+def ref_dm_available():
+    return os.path.exists(REF_DM_SO)
+
+
+_ref_dm = None
+
+
+def ref_dm_run(n, gates, nthreads=1):
+    """The reference density-matrix engine (oracle/_ref/libsvsim_dm_ref.so)
+    on n qubits from |0..0><0..0| (arrays of 2^(2n) entries).
+    Returns (re, im, probs, outcomes)."""
+    global _ref_dm
+    if _ref_dm is None:
+        if not ref_dm_available():
+            raise RuntimeError("oracle/_ref density-matrix engine not built (reference sources absent)")
+        L = ctypes.CDLL(REF_DM_SO)
+        L.ref_qir_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_size_t, _vp, _vp,
+                                  _vp, _vp]
+        L.ref_qir_run.restype = ctypes.c_int
+        _ref_dm = L
+    g = as_qir_gates(gates)
+    cap = 2 * n
+    ore = np.zeros(1 << cap)
+    oim = np.zeros(1 << cap)
+    ore[0] = 1.0
+    nm = int(np.count_nonzero(g["op"] == QIR["Measure"]))
+    probs = np.zeros(max(nm, 1))
+    outs = np.zeros(max(nm, 1), dtype=np.int32)
+    rc = _ref_dm.ref_qir_run(cap, n, nthreads, _ptr(g), len(g), _ptr(ore), _ptr(oim), _ptr(probs), _ptr(outs))
+    if rc != 0:
+        raise ValueError("reference density-matrix driver rejected the gate list")
+    return ore, oim, probs[:nm], outs[:nm]
+

@@ -358,3 +358,134 @@ int orc_qir_apply(int n, double* re, double* im, const orc_qir_gate* gates, size
     free(m);
     return 0;
 }
+
+
+/* ---- density-matrix engine: svsim_qsharp_noise/svsim_cpu_omp.hpp ----
+ * rho of n qubits as a 2n-qubit vector, index = row | (col << n), from
+ * rho = |0><0| (cpu_omp:218-240; update(), 467-475, sizes it 2^(2n)).  Every
+ * gate is U on (q, mask) followed by conj(U) on (q + n, mask << n) (the
+ * *_OP table, 2785-2990: Y -> ConjugateY, S -> AdjointS, RX -> ConjugateRX,
+ * ..., real gates repeated); conj(U) is applied here as conj(U conj(x)),
+ * exact sign flips around the same gate routine.  Swap exchanges the row
+ * qubits only (SWAP_GATE, 1560-1606).  The noise superoperators of
+ * noise_gate_*.cuh are not applied: their calls are commented out at every
+ * call site (2792-2840). */
+
+static void q_conj(qsv_t* s)
+{
+    #pragma omp for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->dim; ii++) s->im[ii] = -s->im[ii];
+}
+
+/* Measure_GATE (1710-1830): basis change on both halves (X: H, H; Y:
+ * AdjointS(q), S(q + n), H, H), m[i] = |Re rho[i, i]| for the rows i < 2^n
+ * with bit q set (0 elsewhere; the rest of m stays zero), halving-tree sum
+ * over all 2^(2n) entries into m[0] = P(1), then with rand < P(1) every
+ * entry whose row bit q is 0 zeroed and the others divided by P(1) (else the
+ * converse with 1 - P(1)) -- the row index only -- and the inverse change
+ * (X: H, H; Y: H, H, S(q), AdjointS(q + n)). */
+static void q_dm_measure(qsv_t* s, int n, int q, double rand_u, uint64_t pauli, double* m, double* p_out)
+{
+    double* R = s->re; double* I = s->im;
+    const uint64_t bit = ONE << q, rows = ONE << n;
+    if (pauli == 1) {
+        q_h(s, q, 0);
+        q_h(s, q + n, 0);
+    }
+    if (pauli == 2) {
+        q_sdg(s, q, 0);
+        q_s(s, q + n, 0);
+        q_h(s, q, 0);
+        q_h(s, q + n, 0);
+    }
+    #pragma omp for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->dim; ii++) {
+        const uint64_t i = (uint64_t)ii;
+        m[i] = (i >= rows || (i & bit) == 0) ? 0. : fabs(R[(i << n) + i]);
+    }
+    for (uint64_t k = s->half_dim; k > 0; k >>= 1) {
+        #pragma omp for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)k; ii++) m[ii] += m[ii + k];
+    }
+    const double p = m[0];
+    #pragma omp barrier
+    const int one = rand_u < p;
+    const double f = one ? p : 1.0 - p;
+    #pragma omp for schedule(static)
+    for (int64_t ii = 0; ii < (int64_t)s->dim; ii++) {
+        const uint64_t i = (uint64_t)ii;
+        if (((i & bit) != 0) == one) {
+            R[i] /= f;
+            I[i] /= f;
+        } else {
+            R[i] = 0.;
+            I[i] = 0.;
+        }
+    }
+    if (pauli == 1) {
+        q_h(s, q, 0);
+        q_h(s, q + n, 0);
+    }
+    if (pauli == 2) {
+        q_h(s, q, 0);
+        q_h(s, q + n, 0);
+        q_s(s, q, 0);
+        q_sdg(s, q + n, 0);
+    }
+    *p_out = p;
+}
+
+int orc_dm_qir_apply(int n, double* re, double* im, const orc_qir_gate* gates, size_t ng, double* probs,
+                     int32_t* outcomes, int nthreads)
+{
+    if (n < 1 || n > 20) return -1;
+    for (size_t i = 0; i < ng; i++) {
+        const orc_qir_gate* g = &gates[i];
+        if (g->op < 0 || g->op >= ORQ_NUM_OPS) return -1;
+        if (g->qubit < 0 || g->qubit >= n) return -1;
+        if (g->op == ORQ_SWAP && (g->mask >= (uint64_t)n || g->mask == (uint64_t)g->qubit)) return -1;
+    }
+    qsv_t s;
+    s.dim = ONE << (2 * n);
+    s.half_dim = s.dim >> 1;
+    s.re = re;
+    s.im = im;
+    double* m = (double*)calloc(s.dim, sizeof(double));
+    if (!m) return -1;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        size_t my_mi = 0;
+        for (size_t i = 0; i < ng; i++) {
+            const orc_qir_gate* g = &gates[i];
+            if (g->op == ORQ_MEASURE) {
+                double p = 0.0;
+                q_dm_measure(&s, n, g->qubit, g->theta, g->mask, m, &p);
+                #pragma omp barrier
+                #pragma omp single
+                {
+                    if (probs) probs[my_mi] = p;
+                    if (outcomes) outcomes[my_mi] = g->theta < p ? 1 : 0;
+                }
+                my_mi++;
+                continue;
+            }
+            if (g->op == ORQ_SWAP) {
+                q_swap(&s, g->qubit, (int)g->mask);
+                continue;
+            }
+            q_exec(&s, g, NULL, NULL);
+            orc_qir_gate c = *g;
+            c.qubit += n;
+            c.mask = g->mask << n;
+            q_conj(&s);
+            q_exec(&s, &c, NULL, NULL);
+            q_conj(&s);
+        }
+    }
+    free(m);
+    return 0;
+}
+

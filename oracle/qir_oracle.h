@@ -62,6 +62,12 @@ typedef struct orc_qir_gate {
 int orc_qir_apply(int n, double* re, double* im, const orc_qir_gate* gates, size_t ng,
                   double* last_prob, double* probs, int32_t* outcomes, int nthreads);
 
+/* The density-matrix engine (svsim_qsharp_noise/svsim_cpu_omp.hpp) on n
+ * qubits: re/im hold rho as 2^(2n) entries, index row | (col << n); same
+ * records, probs/outcomes as above.  n <= 20.  Returns 0 or -1 as above. */
+int orc_dm_qir_apply(int n, double* re, double* im, const orc_qir_gate* gates, size_t ng, double* probs,
+                     int32_t* outcomes, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

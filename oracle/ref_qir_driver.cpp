@@ -1,6 +1,10 @@
 // ref_qir_driver.cpp -- C entry point around the REAL reference QIR-facing
 // engine (svsim_qsharp/svsim_cpu_omp.hpp), compiled straight from the
-// reference sources by oracle/Makefile into oracle/_ref/libsvsim_qir_ref.so.
+// reference sources by oracle/Makefile into oracle/_ref/libsvsim_qir_ref.so;
+// compiled a second time against the density-matrix engine
+// (svsim_qsharp_noise/svsim_cpu_omp.hpp, same factory names) into
+// oracle/_ref/libsvsim_dm_ref.so, where n qubits need capacity >= 2n (its
+// update(), 467-475, runs on 2^(2n) entries).
 //
 // TEST INFRASTRUCTURE ONLY: used to pin the oracle restatement
 // (oracle/qir_oracle.c) and to write the golden fixtures under tests/golden/.
@@ -13,7 +17,10 @@
 // (cpu_omp:258-427); the run is Simulation::sim (467-505), called once per
 // Measure as the QIR wrapper does (svsim_qsharp_wrapper.cpp:541-542).
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <fcntl.h>
+#include <unistd.h>
 
 #include "util_cpu.h"
 #include "svsim_cpu_omp.hpp"
@@ -25,6 +32,14 @@ struct ref_qir_gate {
     int32_t qubit;
     uint64_t mask;
     double theta;
+};
+
+// Silences the per-run banner the density-matrix engine prints from sim()
+// (svsim_qsharp_noise/config.hpp PRINT_MEA_PER_CIRCUIT, cpu_omp:508-514).
+struct StdoutMute {
+    int saved;
+    StdoutMute() { fflush(stdout); saved = dup(1); int nul = open("/dev/null", O_WRONLY); dup2(nul, 1); close(nul); }
+    ~StdoutMute() { fflush(stdout); dup2(saved, 1); close(saved); }
 };
 
 static void add_gate(Simulation& s, const ref_qir_gate& g)
@@ -92,6 +107,7 @@ int ref_qir_run(int capacity, int n, int nthreads, const ref_qir_gate* gates, si
     std::memcpy(s.sv_imag, im, dim * sizeof(double));
     for (int i = 0; i < n; i++) s.AllocateQubit();
     size_t m = 0;
+    StdoutMute mute;
     try {
         for (size_t i = 0; i < ng; i++) {
             add_gate(s, gates[i]);
