@@ -238,6 +238,22 @@ typedef struct svmi_qir_gate {
 svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates,
                          double* probs, int32_t* outcomes, double* last_prob);
 
+/* ---- Simulation::sim of the density-matrix engine
+ *      (svsim_qsharp_noise/svsim_cpu_omp.hpp:480-520): the same gate records
+ *      on rho of n_qubits qubits, held as the first 2^(2 n_qubits)
+ *      amplitudes of the device state (index row | col << n_qubits; the
+ *      handle needs 2 n_qubits qubits; after svmi_reset_state it holds
+ *      rho = |0..0><0..0|).  Each gate acts as U on (qubit, mask) and as
+ *      conj(U) on (qubit + n_qubits, mask << n_qubits) (2785-2990); Swap
+ *      exchanges the row qubits only (1560-1606); Measure changes basis on
+ *      both halves, takes P(1) = sum of |Re rho[i, i]| over the rows with the
+ *      bit set and divides the rows with the outcome's bit by P(1) (or
+ *      1 - P(1)) and zeroes the others, rows only (1710-1830).  The engine's
+ *      noise superoperators are not applied there (their calls are commented
+ *      out, 2792-2840), nor here.  probs / outcomes as above. */
+svmi_status svmi_dm_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates,
+                            double* probs, int32_t* outcomes);
+
 /* ---- OpenQASM 2.0 front end (replaces the translator
  *      svsim/QASMBench/svsim_qasm.py, which writes a Python script of
  *      sim.append(sim.OP(...)) calls).  Host only: parses `text` (len bytes)

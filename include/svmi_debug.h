@@ -54,6 +54,11 @@ svmi_status svmi_debug_emulate_qir(int n, const svmi_qir_gate* gates, size_t n_g
 
 /* Plan a QIR gate list as svmi_qir_run does: sweeps, measurement steps and
  * a text description (truncated to buflen-1 bytes; buf may be NULL). */
+/* svmi_dm_qir_run's program on the host for a density matrix of n <= 13
+ * qubits: re/im hold 2^(2n) entries in and out. */
+svmi_status svmi_debug_emulate_dm_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits,
+                                      int low_qubits, double* re, double* im, double* probs, int32_t* outcomes);
+
 svmi_status svmi_debug_plan_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
                                 uint64_t* n_sweeps, uint64_t* n_measures, char* buf, size_t buflen);
 

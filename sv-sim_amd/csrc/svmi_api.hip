@@ -136,7 +136,7 @@ void plan_forward(const svmi_gate* gates, size_t n_gates, int n, const PlanOptio
 // QIR gate list on n allocated qubits -> segmented program on n_prog >= n
 // qubits (n_prog >= 4: every sweep needs 4 register bits).
 svmi_status plan_qir(const svmi_qir_gate* gates, size_t n_gates, int n, int n_prog, const PlanOptions& opt,
-                     Program& prog)
+                     Program& prog, bool dm = false)
 {
     if (n_gates && !gates) return fail(SVMI_ERR_ARG, "null gate array");
     std::string err;
@@ -146,7 +146,10 @@ svmi_status plan_qir(const svmi_qir_gate* gates, size_t n_gates, int n, int n_pr
     }
     std::vector<std::vector<LOp>> segs;
     std::vector<MeasureStep> meas;
-    lower_qir(gates, n_gates, n, segs, meas);
+    if (dm)
+        lower_dm_qir(gates, n_gates, n, segs, meas);
+    else
+        lower_qir(gates, n_gates, n, segs, meas);
     for (auto& seg : segs) {
         std::vector<LOp> f;
         fuse_single_qubit_runs(seg, f);
@@ -712,7 +715,7 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
             const MeasureStep& m = prog.measures[st.index];
             SVMI_TRY(h->scratch[4].ensure(measure_partials(int(m.n)) * sizeof(double)));
             SVMI_HIP(launch_measure(h->stream, s0, int(m.n), m.qubit, m.draw, h->scratch[4].as<double>(),
-                                    dp.coef.as<c64>() + m.coef, meas + st.index));
+                                    dp.coef.as<c64>() + m.coef, meas + st.index, int(m.dm_n)));
         } else {
             const StandaloneOp& so = prog.standalone[st.index];
             SVMI_HIP(launch_standalone(h->stream, s0, h->n_int, so, dp.coef.as<c64>()));
@@ -1051,13 +1054,16 @@ svmi_status svmi_qasm_parse(const char* text, size_t len, svmi_gate* gates, size
     return SVMI_OK;
 }
 
-svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates, double* probs,
-                         int32_t* outcomes, double* last_prob)
+svmi_status qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates, double* probs,
+                    int32_t* outcomes, double* last_prob, bool dm)
 {
     SVMI_TRY(use_device(h));
-    if (n_qubits < 1 || n_qubits > h->n)
-        return fail(SVMI_ERR_ARG, "n_qubits must be in [1, " + std::to_string(h->n) + "] (the handle's qubits)");
-    SVMI_TRY(plan_qir(gates, n_gates, n_qubits, std::max(n_qubits, kRegBits), h->opt, h->qir_prog));
+    const int n_state = dm ? 2 * n_qubits : n_qubits;
+    if (n_qubits < 1 || n_state > h->n)
+        return fail(SVMI_ERR_ARG, dm ? "density matrix of n_qubits needs a handle of 2 * n_qubits qubits (has " +
+                                           std::to_string(h->n) + ")"
+                                     : "n_qubits must be in [1, " + std::to_string(h->n) + "] (the handle's qubits)");
+    SVMI_TRY(plan_qir(gates, n_gates, n_qubits, std::max(n_state, kRegBits), h->opt, h->qir_prog, dm));
     const Program& prog = h->qir_prog;
     SVMI_TRY(h->qir_dprog.upload(prog, h->stream));
     const size_t nm = prog.measures.size();
@@ -1086,6 +1092,18 @@ svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates
     if (nm) h->qir_last_prob = res[nm - 1].prob;
     if (last_prob) *last_prob = h->qir_last_prob;
     return SVMI_OK;
+}
+
+svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates, double* probs,
+                         int32_t* outcomes, double* last_prob)
+{
+    return qir_run(h, n_qubits, gates, n_gates, probs, outcomes, last_prob, false);
+}
+
+svmi_status svmi_dm_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates, double* probs,
+                            int32_t* outcomes)
+{
+    return qir_run(h, n_qubits, gates, n_gates, probs, outcomes, nullptr, true);
 }
 
 svmi_status svmi_measure(svmi_handle h, const double* uniforms, size_t shots, uint64_t* out, double* total_prob)
@@ -1478,10 +1496,16 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
 MeasureResult host_measure(const MeasureStep& m, const std::vector<c64>& s, std::vector<c64>& coef)
 {
     double p = 0.0;
-    for (uint64_t x = 0; x < (1ull << m.n); x++)
-        if ((x >> m.qubit) & 1) p += s[x].re * s[x].re + s[x].im * s[x].im;
+    if (m.dm_n) {   // density matrix: the diagonal of the rows with bit q set
+        for (uint64_t i = 0; i < (1ull << m.dm_n); i++)
+            if ((i >> m.qubit) & 1) p += std::fabs(s[(i << m.dm_n) | i].re);
+    } else {
+        for (uint64_t x = 0; x < (1ull << m.n); x++)
+            if ((x >> m.qubit) & 1) p += s[x].re * s[x].re + s[x].im * s[x].im;
+    }
     const bool one = m.draw < p;
-    const double f = one ? 1.0 / std::sqrt(p) : 1.0 / std::sqrt(1.0 - p);
+    const double w = one ? p : 1.0 - p;
+    const double f = m.dm_n ? 1.0 / w : 1.0 / std::sqrt(w);
     coef[m.coef] = c64{one ? 0.0 : f, 0.0};
     coef[m.coef + 1] = c64{one ? f : 0.0, 0.0};
     MeasureResult r;
@@ -1744,6 +1768,31 @@ svmi_status svmi_debug_emulate_qir(int n, const svmi_qir_gate* gates, size_t n_g
     std::vector<MeasureResult> res;
     host_program(prog, s, dummy, g, &res);
     for (uint64_t x = 0; x < (1ull << n); x++) {
+        re[x] = s[x].re;
+        im[x] = s[x].im;
+    }
+    for (size_t m = 0; m < res.size(); m++) {
+        if (probs) probs[m] = res[m].prob;
+        if (outcomes) outcomes[m] = res[m].outcome;
+    }
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_emulate_dm_qir(int n, const svmi_qir_gate* gates, size_t n_gates, int tile_qubits,
+                                      int low_qubits, double* re, double* im, double* probs, int32_t* outcomes)
+{
+    if (n < 1 || n > 13) return fail(SVMI_ERR_ARG, "density-matrix emulation supports 1..13 qubits");
+    if (!re || !im) return fail(SVMI_ERR_ARG, "null state");
+    if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
+    const int n2 = 2 * n, ni = std::max(n2, kRegBits);
+    Program prog;
+    SVMI_TRY(plan_qir(gates, n_gates, n, ni, debug_options(1, tile_qubits, low_qubits, 0), prog, true));
+    std::vector<c64> s(1ull << ni, c64{0.0, 0.0}), dummy;
+    for (uint64_t x = 0; x < (1ull << n2); x++) s[x] = c64{re[x], im[x]};
+    std::vector<double> g;
+    std::vector<MeasureResult> res;
+    host_program(prog, s, dummy, g, &res);
+    for (uint64_t x = 0; x < (1ull << n2); x++) {
         re[x] = s[x].re;
         im[x] = s[x].im;
     }

@@ -1098,9 +1098,36 @@ k_prob_one(const c64* __restrict__ s, uint64_t half, uint32_t q, double* __restr
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
+// Density matrix of nh qubits (index row | col << nh): partial sums of
+// |Re rho[i, i]| over the rows i with bit q set -- the diagonal entries only
+// (2^(nh-1) strided reads), as Measure_GATE of the density-matrix engine
+// sums them (svsim_qsharp_noise/svsim_cpu_omp.hpp:1737-1757).
+__global__ void __launch_bounds__(kThreads)
+k_prob_one_dm(const c64* __restrict__ s, uint32_t nh, uint32_t q, double* __restrict__ partials)
+{
+    __shared__ double red[kThreads];
+    const uint64_t G = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t half = 1ull << (nh - 1);
+    const uint64_t lo = (1ull << q) - 1;
+    double acc = 0.0;
+    for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < half; j += G) {
+        const uint64_t i = ((j & ~lo) << 1) | (1ull << q) | (j & lo);
+        acc += fabs(s[(i << nh) | i].re);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// dm: collapse factors 1/P(1) or 1/(1 - P(1)) (density matrix) instead of
+// their square roots (state vector).
 __global__ void __launch_bounds__(kThreads)
 k_measure_decide(const double* __restrict__ partials, uint32_t nparts, double draw, c64* __restrict__ collapse,
-                 MeasureResult* __restrict__ res)
+                 MeasureResult* __restrict__ res, int dm)
 {
     __shared__ double red[kThreads];
     double acc = 0.0;
@@ -1114,7 +1141,8 @@ k_measure_decide(const double* __restrict__ partials, uint32_t nparts, double dr
     if (threadIdx.x == 0) {
         const double p = red[0];
         const bool one = draw < p;
-        const double f = one ? 1.0 / sqrt(p) : 1.0 / sqrt(1.0 - p);
+        const double w = one ? p : 1.0 - p;
+        const double f = dm ? 1.0 / w : 1.0 / sqrt(w);
         collapse[0] = c64{one ? 0.0 : f, 0.0};
         collapse[1] = c64{one ? f : 0.0, 0.0};
         MeasureResult r;
@@ -1133,11 +1161,17 @@ uint32_t measure_partials(int n)
 }
 
 hipError_t launch_measure(hipStream_t st, const c64* s, int n, uint32_t q, double draw, double* partials,
-                          c64* collapse, MeasureResult* res)
+                          c64* collapse, MeasureResult* res, int dm_n)
 {
+    if (dm_n > 0) {
+        const uint32_t g = measure_partials(dm_n);
+        hipLaunchKernelGGL(k_prob_one_dm, dim3(g), dim3(kThreads), 0, st, s, uint32_t(dm_n), q, partials);
+        hipLaunchKernelGGL(k_measure_decide, dim3(1), dim3(kThreads), 0, st, partials, g, draw, collapse, res, 1);
+        return hipGetLastError();
+    }
     const uint32_t g = measure_partials(n);
     hipLaunchKernelGGL(k_prob_one, dim3(g), dim3(kThreads), 0, st, s, 1ull << (n - 1), q, partials);
-    hipLaunchKernelGGL(k_measure_decide, dim3(1), dim3(kThreads), 0, st, partials, g, draw, collapse, res);
+    hipLaunchKernelGGL(k_measure_decide, dim3(1), dim3(kThreads), 0, st, partials, g, draw, collapse, res, 0);
     return hipGetLastError();
 }
 

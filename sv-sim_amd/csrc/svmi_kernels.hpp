@@ -115,9 +115,12 @@ struct MeasureResult {
 // P(1) of qubit q over the 2^n-amplitude state s, the outcome for `draw`,
 // and the collapse factors written into collapse[0] (bit 0) and collapse[1]
 // (bit 1): (0, 1/sqrt(P)) for outcome 1, (1/sqrt(1-P), 0) for outcome 0.
-// partials: measure_partials(n) doubles of scratch.
+// dm_n > 0: s is a density matrix of dm_n qubits (n = 2 dm_n, index row |
+// col << dm_n), P(1) sums |Re rho[i, i]| over the rows with bit q set and the
+// factors are 1/P and 1/(1-P).  partials: measure_partials(n) doubles of
+// scratch.
 uint32_t measure_partials(int n);
 hipError_t launch_measure(hipStream_t st, const c64* s, int n, uint32_t q, double draw, double* partials,
-                          c64* collapse, MeasureResult* res);
+                          c64* collapse, MeasureResult* res, int dm_n = 0);
 
 }  // namespace svmi

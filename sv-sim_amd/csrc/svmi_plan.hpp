@@ -32,6 +32,9 @@ struct MeasureStep {
     uint32_t qubit = 0;
     uint32_t coef = UINT32_MAX;
     uint32_t n = 0;        // P(1) sums over the first 2^n amplitudes
+    uint32_t dm_n = 0;     // > 0: density matrix of dm_n qubits (2^n = 4^dm_n entries, index
+                           // row | col << dm_n): P(1) = sum of |Re rho[i, i]| over rows i with
+                           // bit `qubit` set, collapse factors 1/P(1) or 1/(1 - P(1))
     double draw = 0.0;
 };
 

@@ -154,6 +154,51 @@ void push_basic(std::vector<LOp>& seg, int op, int q)
     }
 }
 
+// One non-Swap, non-Measure record as ops on qubit q with control mask
+// `mask` (controlled ops only); a mask naming q or a bit outside `all` never
+// acts (OP_HEAD_MASK, 606-615) and emits nothing.  conj: the entrywise
+// conjugate of every matrix (the density-matrix engine's column half).
+void lower_gate_qir(std::vector<LOp>& seg, int op, int q, uint64_t mask, double theta, uint64_t all, bool conj)
+{
+    const size_t first = seg.size();
+    uint64_t ctrl = 0;
+    if (is_controlled(op)) {
+        ctrl = mask;
+        if (((ctrl >> q) & 1) || (ctrl & ~all)) return;   // never satisfied: identity
+    }
+    const int b = base_op(op);
+    if (is_diagonal(b)) {
+        c64 d0, d1;
+        diag_of(b, theta, d0, d1);
+        if (!ctrl) {
+            LOp d;
+            d.kind = LOp::DIAG;
+            d.k = 1;
+            d.q[0] = q;
+            d.m = {d0, d1};
+            seg.push_back(d);
+        } else {
+            // controlled diag(d0, d1) = phase d0 on the controls times phase d1/d0
+            // on controls + target
+            c64 r = d1;
+            if (d0.re != 1.0 || d0.im != 0.0) {
+                const int lo = __builtin_ctzll(ctrl);
+                seg.push_back(phase_op(lo, d0, ctrl & (ctrl - 1)));
+                const double den = d0.re * d0.re + d0.im * d0.im;
+                r = c64{(d1.re * d0.re + d1.im * d0.im) / den, (d1.im * d0.re - d1.re * d0.im) / den};
+            }
+            if (!(d0.re == d1.re && d0.im == d1.im)) seg.push_back(phase_op(q, r, ctrl));
+        }
+    } else {
+        c64 m[4];
+        dense_of(b, theta, m);
+        seg.push_back(one_qubit_dense(q, m, ctrl));
+    }
+    if (conj)
+        for (size_t i = first; i < seg.size(); i++)
+            for (c64& v : seg[i].m) v.im = -v.im;
+}
+
 }  // namespace
 
 svmi_status validate_qir_gate(const svmi_qir_gate& g, int n, std::string& err)
@@ -221,39 +266,70 @@ void lower_qir(const svmi_qir_gate* gates, size_t n_gates, int n, std::vector<st
             seg.push_back(op);
             continue;
         }
-        uint64_t ctrl = 0;
-        if (is_controlled(g.op)) {
-            ctrl = g.mask;
-            if (((ctrl >> q) & 1) || (ctrl & ~all)) continue;   // never satisfied: identity
-        }
-        const int b = base_op(g.op);
-        if (is_diagonal(b)) {
-            c64 d0, d1;
-            diag_of(b, g.theta, d0, d1);
-            if (!ctrl) {
-                LOp d;
-                d.kind = LOp::DIAG;
-                d.k = 1;
-                d.q[0] = q;
-                d.m = {d0, d1};
-                seg.push_back(d);
-                continue;
+        lower_gate_qir(seg, g.op, q, g.mask, g.theta, all, false);
+    }
+}
+
+void lower_dm_qir(const svmi_qir_gate* gates, size_t n_gates, int n, std::vector<std::vector<LOp>>& segments,
+                  std::vector<MeasureStep>& measures)
+{
+    segments.assign(1, {});
+    measures.clear();
+    const int n2 = 2 * n;
+    const uint64_t all = n2 >= 64 ? ~uint64_t(0) : ((uint64_t(1) << n2) - 1);
+    for (size_t i = 0; i < n_gates; i++) {
+        const svmi_qir_gate& g = gates[i];
+        std::vector<LOp>& seg = segments.back();
+        const int q = g.qubit;
+        if (g.op == SVMI_QIR_MEASURE) {
+            const uint64_t pauli = g.mask;
+            if (pauli == 1) {
+                push_basic(seg, SVMI_QIR_H, q);
+                push_basic(seg, SVMI_QIR_H, q + n);
             }
-            // controlled diag(d0, d1) = phase d0 on the controls times phase d1/d0
-            // on controls + target
-            c64 r = d1;
-            if (d0.re != 1.0 || d0.im != 0.0) {
-                const int lo = __builtin_ctzll(ctrl);
-                seg.push_back(phase_op(lo, d0, ctrl & (ctrl - 1)));
-                const double den = d0.re * d0.re + d0.im * d0.im;
-                r = c64{(d1.re * d0.re + d1.im * d0.im) / den, (d1.im * d0.re - d1.re * d0.im) / den};
+            if (pauli == 2) {
+                push_basic(seg, SVMI_QIR_ADJOINT_S, q);
+                push_basic(seg, SVMI_QIR_S, q + n);
+                push_basic(seg, SVMI_QIR_H, q);
+                push_basic(seg, SVMI_QIR_H, q + n);
             }
-            if (!(d0.re == d1.re && d0.im == d1.im)) seg.push_back(phase_op(q, r, ctrl));
+            MeasureStep m;
+            m.qubit = uint32_t(q);
+            m.n = uint32_t(n2);
+            m.dm_n = uint32_t(n);
+            m.draw = g.theta;
+            measures.push_back(m);
+            segments.emplace_back();
+            std::vector<LOp>& next = segments.back();
+            LOp col = phase_op(q, c64{1.0, 0.0}, 0);   // collapse of the row qubit; factors set on the device
+            col.dyn = int(measures.size()) - 1;
+            next.push_back(col);
+            if (pauli == 1) {
+                push_basic(next, SVMI_QIR_H, q);
+                push_basic(next, SVMI_QIR_H, q + n);
+            }
+            if (pauli == 2) {
+                push_basic(next, SVMI_QIR_H, q);
+                push_basic(next, SVMI_QIR_H, q + n);
+                push_basic(next, SVMI_QIR_S, q);
+                push_basic(next, SVMI_QIR_ADJOINT_S, q + n);
+            }
             continue;
         }
-        c64 m[4];
-        dense_of(b, g.theta, m);
-        seg.push_back(one_qubit_dense(q, m, ctrl));
+        if (g.op == SVMI_QIR_SWAP) {   // the row qubits only
+            LOp op;
+            op.kind = LOp::DENSE;
+            op.k = 2;
+            op.q[0] = q;
+            op.q[1] = int(g.mask);
+            op.m.assign(16, c64{0.0, 0.0});
+            op.m[0 * 4 + 0] = op.m[1 * 4 + 2] = op.m[2 * 4 + 1] = op.m[3 * 4 + 3] = c64{1.0, 0.0};
+            seg.push_back(op);
+            continue;
+        }
+        lower_gate_qir(seg, g.op, q, g.mask, g.theta, all, false);
+        const uint64_t cmask = is_controlled(g.op) ? (n >= 64 ? 0 : g.mask << n) : 0;
+        lower_gate_qir(seg, g.op, q + n, cmask, g.theta, all, true);
     }
 }
 

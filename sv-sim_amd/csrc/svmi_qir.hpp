@@ -28,4 +28,15 @@ svmi_status validate_qir_gate(const svmi_qir_gate& g, int n, std::string& err);
 void lower_qir(const svmi_qir_gate* gates, size_t n_gates, int n, std::vector<std::vector<LOp>>& segments,
                std::vector<MeasureStep>& measures);
 
+// The same gate list for the density-matrix engine
+// (svsim_qsharp_noise/svsim_cpu_omp.hpp): rho of n qubits as a 2n-qubit
+// state, index row | (col << n).  Every gate is lowered as U on (q, mask)
+// followed by conj(U) on (q + n, mask << n) (the *_OP table, 2785-2990);
+// Swap exchanges row qubits only (SWAP_GATE, 1560-1606); a measurement
+// changes basis on both halves (X: H, H; Y: AdjointS, S, H, H), takes P(1)
+// from the diagonal (MeasureStep::dm_n) and collapses the row qubit only
+// with 1/P(1) (Measure_GATE, 1710-1830).
+void lower_dm_qir(const svmi_qir_gate* gates, size_t n_gates, int n, std::vector<std::vector<LOp>>& segments,
+                  std::vector<MeasureStep>& measures);
+
 }  // namespace svmi

@@ -52,7 +52,7 @@ EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version", "svmi_qir_run", "svmi_qasm_parse",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
            "svmi_debug_plan_batch", "svmi_debug_plan_adjoint", "svmi_debug_emulate_qir", "svmi_debug_plan_qir",
-           "svmi_debug_jit"]
+           "svmi_debug_jit", "svmi_dm_qir_run", "svmi_debug_emulate_dm_qir"]
 
 
 class SvmiError(RuntimeError):
@@ -130,6 +130,9 @@ def lib():
         L.svmi_debug_plan_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _sz, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz]
         L.svmi_qir_run.argtypes = [_vp, ctypes.c_int, _vp, _sz, _vp, _vp, _vp]
+        L.svmi_dm_qir_run.argtypes = [_vp, ctypes.c_int, _vp, _sz, _vp, _vp]
+        L.svmi_debug_emulate_dm_qir.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                                _vp]
         L.svmi_debug_jit.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64),
                                      ctypes.c_char_p, _sz]
@@ -292,6 +295,17 @@ class Engine:
                                   ctypes.byref(last)))
         return probs[:nm], outs[:nm], last.value
 
+    def dm_qir_run(self, n_qubits, gates):
+        """QIR gates on the density matrix of n_qubits held in the first
+        2^(2 n_qubits) amplitudes (svmi_dm_qir_run; the engine needs 2 n_qubits
+        qubits).  Returns (probs, outcomes) per Measure."""
+        g = as_qir_gates(gates)
+        nm = int(np.count_nonzero(g["op"] == QIR_OP["Measure"]))
+        probs = np.zeros(max(nm, 1))
+        outs = np.zeros(max(nm, 1), dtype=np.int32)
+        _check(lib().svmi_dm_qir_run(self._h, int(n_qubits), _ptr(g), len(g), _ptr(probs), _ptr(outs)))
+        return probs[:nm], outs[:nm]
+
     def set_option(self, option, value):
         _check(lib().svmi_set_option(self._h, int(option), int(value)))
 
@@ -433,6 +447,27 @@ def emulate_qir(n, gates, state, tile_qubits=12, low_qubits=4):
     outs = np.zeros(max(nm, 1), dtype=np.int32)
     _check(lib().svmi_debug_emulate_qir(int(n), _ptr(g), len(g), int(tile_qubits), int(low_qubits), _ptr(re),
                                         _ptr(im), _ptr(probs), _ptr(outs)))
+    return re, im, probs[:nm], outs[:nm]
+
+
+def emulate_dm_qir(n, gates, state=None, tile_qubits=12, low_qubits=4):
+    """Host execution of svmi_dm_qir_run's program on a density matrix of n
+    qubits (2^(2n) entries, default |0..0><0..0|).  Returns (re, im, probs,
+    outcomes)."""
+    g = as_qir_gates(gates)
+    dim = 1 << (2 * n)
+    if state is None:
+        re = np.zeros(dim)
+        im = np.zeros(dim)
+        re[0] = 1.0
+    else:
+        re = np.array(state[0], dtype=np.float64, copy=True)
+        im = np.array(state[1], dtype=np.float64, copy=True)
+    nm = int(np.count_nonzero(g["op"] == QIR_OP["Measure"]))
+    probs = np.zeros(max(nm, 1))
+    outs = np.zeros(max(nm, 1), dtype=np.int32)
+    _check(lib().svmi_debug_emulate_dm_qir(int(n), _ptr(g), len(g), int(tile_qubits), int(low_qubits), _ptr(re),
+                                           _ptr(im), _ptr(probs), _ptr(outs)))
     return re, im, probs[:nm], outs[:nm]
 
 

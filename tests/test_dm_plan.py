@@ -1,0 +1,43 @@
+"""svmi_dm_qir_run's program (density-matrix engine on the sweep machinery),
+executed on the host by svmi_debug_emulate_dm_qir with the kernels' op
+code, against the oracle restatement (itself bit-exact against the
+reference engine, tests/test_dm_oracle.py) and the reference's golden
+program."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import svmi
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+sys.path.insert(0, GOLD)
+from make_qir_golden import random_program  # noqa: E402
+
+
+def test_golden_program():
+    d = np.load(os.path.join(GOLD, "dm_program_n4.npz"))
+    re, im, probs, outs = svmi.emulate_dm_qir(4, d["gates"])
+    assert max(np.abs(re - d["re"]).max(), np.abs(im - d["im"]).max()) < 1e-13
+    assert np.abs(probs - d["probs"]).max() < 1e-13 and np.array_equal(outs, d["outcomes"])
+
+
+@pytest.mark.parametrize("tile,low", [(12, 4), (6, 2), (9, 0)])
+def test_random_programs_vs_oracle(tile, low):
+    rng = np.random.default_rng(60 + tile)
+    for _ in range(10):
+        n = int(rng.integers(2, 7))
+        g = random_program(n, 50, 3, rng)
+        re, im, probs, outs = svmi.emulate_dm_qir(n, g, tile_qubits=tile, low_qubits=low)
+        ore, oim, oprobs, oouts = O.dm_qir_apply(n, g)
+        assert max(np.abs(re - ore).max(), np.abs(im - oim).max()) < 1e-12
+        assert np.abs(probs - oprobs).max() < 1e-12 and np.array_equal(outs, oouts)
+
+
+def test_limits():
+    with pytest.raises(svmi.SvmiError):
+        svmi.emulate_dm_qir(14, [(O.QIR["H"], 0, 0, 0.0)])
+    with pytest.raises(svmi.SvmiError):
+        svmi.emulate_dm_qir(2, [(O.QIR["Swap"], 0, 2, 0.0)])   # second qubit not allocated
