@@ -18,7 +18,8 @@ program with multi-controlled gates and 16 mid-circuit measurements
 (gates/s, sweep and measurement GB/s, and the CPU oracle on a prefix of it),
 the largest QASMBench circuits through the OpenQASM front end, and the u3
 and QFT circuits again with sweeps compiled into straight-line kernels at
-upload (SVMI_OPT_SPECIALIZE; compile time reported beside the run time).
+upload (SVMI_OPT_SPECIALIZE; compile time reported beside the run time),
+and the density-matrix engine on 15 qubits (a 30-qubit state).
 """
 import argparse
 import json
@@ -242,6 +243,61 @@ def bench_qir(n=30, reps=3):
 QASM_CIRCUITS = ("qft_n20", "square_root_n18", "multiplier_n25", "swap_test_n25", "ising_n26", "wstate_n27")
 
 
+def bench_dm(n=15, reps=3):
+    """Density-matrix engine (svmi_dm_qir_run): the QIR gate-set program on
+    rho of n = 15 qubits (a 30-qubit, 16 GiB state), 600 records with 8
+    mid-circuit measurements, from |0..0><0..0|.  Wall time per program and,
+    with HIP events per step, the average sweep and measurement durations."""
+    import svmi
+    from svmi import workloads as W
+    g = W.qir_mixed(n=n, n_gates=600, n_measures=8)
+    e = svmi.Engine(2 * n)
+    e.dm_qir_run(n, g)
+    walls, sweep_ms, sweep_n, meas_ms, meas_n = [], 0.0, 0, 0.0, 0
+    for prof in (0, 1):
+        e.set_option(svmi.OPT_PROFILE, prof)
+        for _ in range(reps):
+            e.reset_state()
+            t0 = time.perf_counter()
+            probs, outs = e.dm_qir_run(n, g)
+            t = time.perf_counter() - t0
+            st = e.stats()
+            if prof:
+                sweep_ms += st["sweep_ms"]
+                sweep_n += st["sweep_launches"]
+                meas_ms += st["other_ms"]
+                meas_n += st["other_launches"]
+            else:
+                walls.append(t)
+    e.close()
+    w = float(np.median(walls))
+    avg_sweep = sweep_ms / max(sweep_n, 1) / 1e3
+    return {"qubits": n, "state_qubits": 2 * n, "records": len(g), "measures": len(probs),
+            "ms_per_program": w * 1e3, "records_per_s": len(g) / w, "sweeps": sweep_n // reps,
+            "sweep_ms": avg_sweep * 1e3, "sweep_GBps": 32.0 * (1 << (2 * n)) / avg_sweep / 1e9,
+            "measure_ms": meas_ms / max(meas_n, 1), "ones": int(outs.sum())}
+
+
+def cpu_baseline_dm(n=15, budget_s=20.0, threads=None):
+    """Oracle restatement of the reference density-matrix engine (bit-exact
+    to it) on the first records of the same program: one timed call on a
+    prefix sized from a one-record calibration to ~budget_s.  (The reference
+    engine itself stops at 13 qubits: 2n < N_QUBIT_SLOT = 28.)"""
+    from oracle import oracle as O
+    from svmi import workloads as W
+    threads = threads or min(16, os.cpu_count() or 1)
+    g = W.qir_mixed(n=n, n_gates=600, n_measures=8)
+    t0 = time.perf_counter()
+    O.dm_qir_apply(n, g[:1], nthreads=threads)
+    t1 = time.perf_counter() - t0
+    k = int(max(2, min(len(g), budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    O.dm_qir_apply(n, g[:k], nthreads=threads)
+    t = time.perf_counter() - t0
+    return {"value": k / t, "unit": "records/s", "cores": threads, "kind": "port",
+            "sample": f"first {k} of {len(g)} records of the {n}-qubit density-matrix program, {t:.1f} s"}
+
+
 def bench_qasm(reps=3):
     """The reference's largest QASMBench circuits (tests/golden/qasm/):
     OpenQASM parse time on the host, then median device time per run."""
@@ -385,7 +441,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
-    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm", "specialized"),
+    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm", "specialized", "dm"),
                     help="run only this extra workload and print its result (profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (HBM counter calibration)")
@@ -401,7 +457,8 @@ def main():
     if a.only:
         fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch,
               "qir": lambda: bench_qir(30, reps=a.steps), "qasm": lambda: bench_qasm(reps=a.steps),
-              "specialized": lambda: bench_specialized(steps=a.steps)}[a.only]
+              "specialized": lambda: bench_specialized(steps=a.steps),
+              "dm": lambda: bench_dm(15, reps=a.steps)}[a.only]
         log(f"[bench] only {a.only}")
         print(json.dumps({a.only: fn()}), flush=True)
         return
@@ -434,6 +491,11 @@ def main():
         if not a.no_cpu:
             extra["qir_mixed_n30"]["cpu_baseline"] = cpu_baseline_qir(30)
             log(f"[bench] qir cpu {extra['qir_mixed_n30']['cpu_baseline']}")
+        log("[bench] density-matrix engine, 15 qubits (30-qubit state), 8 measurements")
+        extra["dm_qir_n15"] = bench_dm(15)
+        if not a.no_cpu:
+            extra["dm_qir_n15"]["cpu_baseline"] = cpu_baseline_dm(15)
+        log(f"[bench] {extra['dm_qir_n15']}")
         log("[bench] QASMBench circuits (OpenQASM front end)")
         extra["qasmbench"] = bench_qasm()
         if not a.no_cpu:
