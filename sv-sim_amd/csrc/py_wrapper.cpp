@@ -4,11 +4,12 @@
 // gate factories.  Additions (no reference counterpart): the OP enum,
 // get_sv/set_sv, measure_with(draws), expval, adjoint, last_sim_ms.
 //
-// QirSimulator: the QIR runtime driver (svsim_qir.hpp, the reference's
-// SVSimSimulator in svsim_qsharp/svsim_qsharp_wrapper.cpp) with the same
-// method names; qubits and results are integer handles, qubit and Pauli
-// arrays are Python sequences.  A QirBackend subclass written in Python can
-// stand in for the GPU engine.
+// QirSimulator / QirNoiseSimulator: the QIR runtime drivers (svsim_qir.hpp,
+// the reference's SVSimSimulator in svsim_qsharp/svsim_qsharp_wrapper.cpp
+// and in svsim_qsharp_noise/svsim_qsharp_wrapper.cpp) with the same method
+// names; qubits and results are integer handles, qubit and Pauli arrays are
+// Python sequences.  A QirBackend subclass written in Python can stand in for
+// the GPU engine.
 #include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -66,6 +67,85 @@ std::vector<qir::Qubit> to_qs(const std::vector<uint64_t>& v)
     std::vector<qir::Qubit> out;
     for (uint64_t x : v) out.push_back(to_q(x));
     return out;
+}
+
+// QirSimulator / QirNoiseSimulator: one driver class's methods.
+template <class QS>
+py::class_<QS> bind_driver(py::module_& m, const char* name)
+{
+    auto ctl = [](void (QS::*f)(long, qir::Qubit[], qir::Qubit)) {
+        return [f](QS& s, const std::vector<uint64_t>& c, uint64_t t) {
+            std::vector<qir::Qubit> cs = to_qs(c);
+            (s.*f)(long(cs.size()), cs.data(), to_q(t));
+        };
+    };
+    auto one = [](void (QS::*f)(qir::Qubit)) { return [f](QS& s, uint64_t t) { (s.*f)(to_q(t)); }; };
+    py::class_<QS> cls(m, name);
+    cls.def(py::init<int, int>(), py::arg("capacity") = 28, py::arg("device") = 0)
+        .def(py::init([](py::object backend) { return new QS(std::unique_ptr<qir::Backend>(new HeldBackend(backend))); }),
+             py::arg("backend"))
+        .def("set_draw", &QS::set_draw)
+        .def("AllocateQubit", [](QS& s) { return from_q(s.AllocateQubit()); })
+        .def("ReleaseQubit", [](QS& s, uint64_t q) { s.ReleaseQubit(to_q(q)); })
+        .def("QubitToString", [](QS& s, uint64_t q) { return s.QubitToString(to_q(q)); })
+        .def("ReleaseResult", [](QS& s, uint64_t r) { s.ReleaseResult(to_r(r)); })
+        .def("AreEqualResults", [](QS& s, uint64_t a, uint64_t b) { return s.AreEqualResults(to_r(a), to_r(b)); })
+        .def("GetResultValue", [](QS& s, uint64_t r) { return s.GetResultValue(to_r(r)); })
+        .def("UseZero", [](QS& s) { return from_r(s.UseZero()); })
+        .def("UseOne", [](QS& s) { return from_r(s.UseOne()); })
+        .def("X", one(&QS::X))
+        .def("Y", one(&QS::Y))
+        .def("Z", one(&QS::Z))
+        .def("H", one(&QS::H))
+        .def("S", one(&QS::S))
+        .def("T", one(&QS::T))
+        .def("AdjointS", one(&QS::AdjointS))
+        .def("AdjointT", one(&QS::AdjointT))
+        .def("R", [](QS& s, qir::PauliId axis, uint64_t t, double theta) { s.R(axis, to_q(t), theta); })
+        .def("Exp", [](QS& s, std::vector<qir::PauliId> paulis, const std::vector<uint64_t>& targets, double theta) {
+            if (paulis.size() != targets.size()) throw std::runtime_error("Exp: paulis and targets differ in length");
+            std::vector<qir::Qubit> ts = to_qs(targets);
+            s.Exp(long(ts.size()), paulis.data(), ts.data(), theta);
+        })
+        .def("ControlledX", ctl(&QS::ControlledX))
+        .def("ControlledY", ctl(&QS::ControlledY))
+        .def("ControlledZ", ctl(&QS::ControlledZ))
+        .def("ControlledH", ctl(&QS::ControlledH))
+        .def("ControlledS", ctl(&QS::ControlledS))
+        .def("ControlledT", ctl(&QS::ControlledT))
+        .def("ControlledAdjointS", ctl(&QS::ControlledAdjointS))
+        .def("ControlledAdjointT", ctl(&QS::ControlledAdjointT))
+        .def("ControlledR", [](QS& s, const std::vector<uint64_t>& c, qir::PauliId axis, uint64_t t, double theta) {
+            std::vector<qir::Qubit> cs = to_qs(c);
+            s.ControlledR(long(cs.size()), cs.data(), axis, to_q(t), theta);
+        })
+        .def("ControlledExp", [](QS& s, const std::vector<uint64_t>& c, std::vector<qir::PauliId> paulis,
+                                 const std::vector<uint64_t>& targets, double theta) {
+            if (paulis.size() != targets.size())
+                throw std::runtime_error("ControlledExp: paulis and targets differ in length");
+            std::vector<qir::Qubit> cs = to_qs(c), ts = to_qs(targets);
+            s.ControlledExp(long(cs.size()), cs.data(), long(ts.size()), paulis.data(), ts.data(), theta);
+        })
+        .def("Measure", [](QS& s, std::vector<qir::PauliId> bases, const std::vector<uint64_t>& targets) {
+            std::vector<qir::Qubit> ts = to_qs(targets);
+            return from_r(s.Measure(long(bases.size()), bases.data(), long(ts.size()), ts.data()));
+        })
+        .def("num_qubits", &QS::num_qubits)
+        .def("slot_of", [](const QS& s, uint64_t q) { return s.slot_of(to_q(q)); })
+        .def("last_prob", &QS::last_prob)
+        .def("pending", &QS::pending)
+        .def("flush", &QS::flush)
+        .def("get_sv", [](QS& s) {
+            svmi_handle h = nullptr;
+            if (auto* d = dynamic_cast<qir::DeviceBackend*>(&s.backend())) h = d->handle();
+            if (auto* d = dynamic_cast<qir::DmDeviceBackend*>(&s.backend())) h = d->handle();
+            if (!h) throw std::runtime_error("get_sv: the simulator does not run on the GPU engine");
+            const size_t dim = size_t(1) << s.backend().capacity();
+            py::array_t<double> re(dim), im(dim);
+            svmi_throw(svmi_get_state(h, re.mutable_data(), im.mutable_data()), "svmi_get_state");
+            return py::make_tuple(re, im);
+        });
+    return cls;
 }
 
 }  // namespace
@@ -195,75 +275,7 @@ PYBIND11_MODULE(libsvsim, m)
         .def("reset", &qir::Backend::reset)
         .def("capacity", &qir::Backend::capacity);
 
-    using QS = qir::SVSimSimulator;
-    auto ctl = [](void (QS::*f)(long, qir::Qubit[], qir::Qubit)) {
-        return [f](QS& s, const std::vector<uint64_t>& c, uint64_t t) {
-            std::vector<qir::Qubit> cs = to_qs(c);
-            (s.*f)(long(cs.size()), cs.data(), to_q(t));
-        };
-    };
-    auto one = [](void (QS::*f)(qir::Qubit)) { return [f](QS& s, uint64_t t) { (s.*f)(to_q(t)); }; };
-    py::class_<QS>(m, "QirSimulator")
-        .def(py::init<int, int>(), py::arg("capacity") = 28, py::arg("device") = 0)
-        .def(py::init([](py::object backend) { return new QS(std::unique_ptr<qir::Backend>(new HeldBackend(backend))); }),
-             py::arg("backend"))
-        .def("set_draw", &QS::set_draw)
-        .def("AllocateQubit", [](QS& s) { return from_q(s.AllocateQubit()); })
-        .def("ReleaseQubit", [](QS& s, uint64_t q) { s.ReleaseQubit(to_q(q)); })
-        .def("QubitToString", [](QS& s, uint64_t q) { return s.QubitToString(to_q(q)); })
-        .def("ReleaseResult", [](QS& s, uint64_t r) { s.ReleaseResult(to_r(r)); })
-        .def("AreEqualResults", [](QS& s, uint64_t a, uint64_t b) { return s.AreEqualResults(to_r(a), to_r(b)); })
-        .def("GetResultValue", [](QS& s, uint64_t r) { return s.GetResultValue(to_r(r)); })
-        .def("UseZero", [](QS& s) { return from_r(s.UseZero()); })
-        .def("UseOne", [](QS& s) { return from_r(s.UseOne()); })
-        .def("X", one(&QS::X))
-        .def("Y", one(&QS::Y))
-        .def("Z", one(&QS::Z))
-        .def("H", one(&QS::H))
-        .def("S", one(&QS::S))
-        .def("T", one(&QS::T))
-        .def("AdjointS", one(&QS::AdjointS))
-        .def("AdjointT", one(&QS::AdjointT))
-        .def("R", [](QS& s, qir::PauliId axis, uint64_t t, double theta) { s.R(axis, to_q(t), theta); })
-        .def("Exp", [](QS& s, std::vector<qir::PauliId> paulis, const std::vector<uint64_t>& targets, double theta) {
-            if (paulis.size() != targets.size()) throw std::runtime_error("Exp: paulis and targets differ in length");
-            std::vector<qir::Qubit> ts = to_qs(targets);
-            s.Exp(long(ts.size()), paulis.data(), ts.data(), theta);
-        })
-        .def("ControlledX", ctl(&QS::ControlledX))
-        .def("ControlledY", ctl(&QS::ControlledY))
-        .def("ControlledZ", ctl(&QS::ControlledZ))
-        .def("ControlledH", ctl(&QS::ControlledH))
-        .def("ControlledS", ctl(&QS::ControlledS))
-        .def("ControlledT", ctl(&QS::ControlledT))
-        .def("ControlledAdjointS", ctl(&QS::ControlledAdjointS))
-        .def("ControlledAdjointT", ctl(&QS::ControlledAdjointT))
-        .def("ControlledR", [](QS& s, const std::vector<uint64_t>& c, qir::PauliId axis, uint64_t t, double theta) {
-            std::vector<qir::Qubit> cs = to_qs(c);
-            s.ControlledR(long(cs.size()), cs.data(), axis, to_q(t), theta);
-        })
-        .def("ControlledExp", [](QS& s, const std::vector<uint64_t>& c, std::vector<qir::PauliId> paulis,
-                                 const std::vector<uint64_t>& targets, double theta) {
-            if (paulis.size() != targets.size())
-                throw std::runtime_error("ControlledExp: paulis and targets differ in length");
-            std::vector<qir::Qubit> cs = to_qs(c), ts = to_qs(targets);
-            s.ControlledExp(long(cs.size()), cs.data(), long(ts.size()), paulis.data(), ts.data(), theta);
-        })
-        .def("Measure", [](QS& s, std::vector<qir::PauliId> bases, const std::vector<uint64_t>& targets) {
-            std::vector<qir::Qubit> ts = to_qs(targets);
-            return from_r(s.Measure(long(bases.size()), bases.data(), long(ts.size()), ts.data()));
-        })
-        .def("num_qubits", &QS::num_qubits)
-        .def("slot_of", [](const QS& s, uint64_t q) { return s.slot_of(to_q(q)); })
-        .def("last_prob", &QS::last_prob)
-        .def("pending", &QS::pending)
-        .def("flush", &QS::flush)
-        .def("get_sv", [](QS& s) {
-            auto* d = dynamic_cast<qir::DeviceBackend*>(&s.backend());
-            if (!d) throw std::runtime_error("get_sv: the simulator does not run on the GPU engine");
-            const size_t dim = size_t(1) << d->capacity();
-            py::array_t<double> re(dim), im(dim);
-            svmi_throw(svmi_get_state(d->handle(), re.mutable_data(), im.mutable_data()), "svmi_get_state");
-            return py::make_tuple(re, im);
-        });
+    bind_driver<qir::SVSimSimulator>(m, "QirSimulator");
+    bind_driver<qir::noise::SVSimSimulator>(m, "QirNoiseSimulator")
+        .def("engine_qubits", &qir::noise::SVSimSimulator::engine_qubits);
 }

@@ -67,3 +67,34 @@ def test_n14_diagonal_matches_state_vector_engine():
     want = (pre[3] + 1j * pim[3]) * np.conj(pre[5] + 1j * pim[5])   # rho[row 3, col 5]
     assert abs(off[0][0] + 1j * off[1][0] - want) < 1e-12
     assert abs(dre.sum() - 1.0) < 1e-10
+
+
+def test_qir_noise_driver_on_the_device_vs_the_oracle_engine():
+    """libsvsim.QirNoiseSimulator on the GPU engine against the same driver on
+    the oracle engine: the same calls and draws, including a release and a
+    re-allocation that grow the engine's qubit count (rho reinterpreted over
+    the same amplitudes, as in the reference)."""
+    import libsvsim
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_qir_noise_driver import OracleDmBackend, random_calls
+    P = libsvsim.PauliId
+    cap = 10
+    be = OracleDmBackend(cap)
+    sims = [libsvsim.QirNoiseSimulator(capacity=cap, device=0), libsvsim.QirNoiseSimulator(backend=be)]
+    results = [[], []]
+    for i, sim in enumerate(sims):
+        draws = iter(np.random.default_rng(31).random(400))
+        sim.set_draw(lambda d=draws: float(next(d)))
+        qs = [sim.AllocateQubit() for _ in range(4)]
+        random_calls(sim, qs, np.random.default_rng(32), 120, measure=True)
+        results[i].append(sim.GetResultValue(sim.Measure([P.PauliId_Y], [qs[1]])))
+        sim.ReleaseQubit(qs[3])
+        qs[3] = sim.AllocateQubit()
+        assert sim.engine_qubits() == 5
+        random_calls(sim, qs, np.random.default_rng(33), 60, measure=True)
+        results[i].append(sim.GetResultValue(sim.Measure([P.PauliId_X], [qs[0]])))
+        results[i].append(sim.last_prob())
+    print("case: noise driver, 4 live qubits, engine 4 then 5", flush=True)
+    re, im = sims[0].get_sv()
+    assert results[0][:2] == results[1][:2] and abs(results[0][2] - results[1][2]) < TOL
+    assert max(np.abs(re - be.re).max(), np.abs(im - be.im).max()) < TOL
