@@ -10,6 +10,8 @@
 // names; qubits and results are integer handles, qubit and Pauli arrays are
 // Python sequences.  A QirBackend subclass written in Python can stand in for
 // the GPU engine.
+#include <memory>
+
 #include <pybind11/functional.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -171,9 +173,9 @@ PYBIND11_MODULE(libsvsim, m)
         .def("run", &Simulation::sim, py::call_guard<py::gil_scoped_release>())
         .def("reset", &Simulation::reset)
         .def("measure", [](Simulation& s, unsigned repetition) {
-            std::vector<IdxType> r = s.measure(repetition);
+            std::unique_ptr<IdxType[]> r(s.measure(repetition));
             py::list out;
-            for (IdxType v : r) out.append(v);
+            for (unsigned i = 0; i < repetition; i++) out.append(r[i]);
             return out;
         })
         .def("measure_with", [](Simulation& s, py::array_t<double, py::array::c_style | py::array::forcecast> u) {

@@ -11,6 +11,7 @@
 //     sv_real/sv_imag members, and sim() makes no host copy.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -169,12 +170,17 @@ public:
     }
     // svsim_cpu_omp.hpp:339-361: draws r = rand()/RAND_MAX after
     // srand(time(0)), the reference's RAND_SEED (config.hpp:44).
-    std::vector<IdxType> measure(unsigned repetition = 10)
+    // Returns new IdxType[repetition], owned by the caller (delete[]), as
+    // the reference's measure returns its array.
+    IdxType* measure(unsigned repetition = 10)
     {
         std::vector<double> u(repetition);
         srand(unsigned(time(0)));
         for (unsigned i = 0; i < repetition; i++) u[i] = double(rand()) / double(RAND_MAX);
-        return measure_with(u);
+        const std::vector<IdxType> r = measure_with(u);
+        IdxType* res_state = new IdxType[repetition];
+        std::copy(r.begin(), r.end(), res_state);
+        return res_state;
     }
     // measure with caller-supplied draws (same interval rule)
     std::vector<IdxType> measure_with(const std::vector<double>& uniforms)
@@ -255,5 +261,27 @@ public:
 private:
     svmi_handle h = nullptr;
 };
+
+// svsim/src/util_cpu.h:70-85: a basis-state index as n_qubits binary digits
+// (qubit n_qubits-1 first), and one line per measured state.
+inline void print_binary(IdxType v, int width)
+{
+    for (int i = width - 1; i >= 0; i--) putchar('0' + int((v >> i) & 1));
+}
+inline void print_measurement(const IdxType* res_state, IdxType n_qubits, int repetition)
+{
+    if (!res_state) throw std::runtime_error("print_measurement: null results");
+    printf("\n===============  Measurement (tests=%d) ================\n", repetition);
+    for (int i = 0; i < repetition; i++) {
+        printf("Test-%d: ", i);
+        print_binary(res_state[i], int(n_qubits));
+        printf("\n");
+    }
+}
+inline void print_measurement(const std::vector<IdxType>& res_state, IdxType n_qubits, int repetition)
+{
+    if (res_state.size() < size_t(repetition)) throw std::runtime_error("print_measurement: too few results");
+    print_measurement(res_state.data(), n_qubits, repetition);
+}
 
 }  // namespace SVSim

@@ -11,7 +11,14 @@ The command line mirrors the reference translator svsim/QASMBench/svsim_qasm.py:
 
 writes a script for the `libsvsim` module (`python svsim_circuit.py n_qubits
 gpu`) with one `sim.append(sim.OP(...))` per gate, and prints the qubit,
-basic-gate and CX counts as the reference does.
+basic-gate and CX counts as the reference does.  With an output ending in
+.cpp/.cc/.cxx (or --cplus) it writes, as svsim/tool/svsim_qasm_cplus.py
+does, a C++ program for the `SVSim::Simulation` facade (svsim_amd.hpp): one
+`Simulation::OP(...)` append per gate in prepare_circuit, then a main that
+uploads, runs and prints 5 measured states:
+
+    python -m svmi.qasm -i adder.qasm -o adder.cpp
+    g++ -std=c++17 -O2 adder.cpp -I include -I sv-sim_amd/csrc -L sv-sim_amd -lsvmi
 """
 import argparse
 import ctypes
@@ -70,23 +77,53 @@ def to_script(gates, comment=""):
     out += ["if len(sys.argv) != 3:", "    print('$python circuit.py n_qubits gpu')", "    exit()", "",
             "sim = svsim.Simulation(int(sys.argv[1]), int(sys.argv[2]))", ""]
     for g in gates:
-        name = OP_NAMES[int(g["op"])]
-        args = [repr(float(g[a])) for a in _ANGLES.get(name, ())]
-        args += [str(int(q)) for q in g["qb"][:_ARITY.get(name, 1)]]
+        name, args = _gate_args(g)
         out.append(f"sim.append(sim.{name}({', '.join(args)}))")
     out += ["", "sim.upload()", "sim.run()", ""]
     return "\n".join(out)
 
 
+def _gate_args(g):
+    name = OP_NAMES[int(g["op"])]
+    args = [repr(float(g[a])) for a in _ANGLES.get(name, ())]
+    args += [str(int(q)) for q in g["qb"][:_ARITY.get(name, 1)]]
+    return name, args
+
+
+def to_cplus(n_qubits, gates, comment=""):
+    """C++ source for the SVSim::Simulation facade, one append per gate
+    (the program svsim/tool/svsim_qasm_cplus.py:346-372 writes)."""
+    out = ["#include <cstdio>", '#include "svsim_amd.hpp"', ""]
+    if comment:
+        out += ["// " + line for line in comment.splitlines()] + [""]
+    out += ["// Use the SVSim namespace to enable the C++ API", "using namespace SVSim;", "",
+            "void prepare_circuit(Simulation& sim)", "{",
+            "    auto add = [&sim](Gate* g) { sim.append(g); delete g; };"]
+    for g in gates:
+        name, args = _gate_args(g)
+        out.append(f"    add(Simulation::{name}({', '.join(args)}));")
+    out += ["}", "", "int main()", "{", f"    int n_qubits = {int(n_qubits)};", "    int i_gpu = 0;",
+            "    Simulation sim(n_qubits, i_gpu);", "    prepare_circuit(sim);", "    sim.upload();", "    sim.sim();",
+            "    auto* res = sim.measure(5);", "    print_measurement(res, n_qubits, 5);", "    delete[] res;",
+            "    return 0;", "}", ""]
+    return "\n".join(out)
+
+
 def main(argv=None):
-    ap = argparse.ArgumentParser(description="OpenQASM 2.0 -> libsvsim circuit script (svsim_qasm.py's interface)")
+    ap = argparse.ArgumentParser(description="OpenQASM 2.0 -> libsvsim circuit script or C++ program "
+                                             "(svsim_qasm.py's and svsim_qasm_cplus.py's interface)")
     ap.add_argument("--input", "-i", required=True, help="input OpenQASM file, such as adder.qasm")
-    ap.add_argument("--output", "-o", default="svsim_circuit.py", help="output script (default: svsim_circuit.py)")
+    ap.add_argument("--output", "-o", default=None,
+                    help="output script (default: svsim_circuit.py, or svsim_circuit.cpp with --cplus)")
+    ap.add_argument("--cplus", action="store_true", help="write a C++ program (implied by a .cpp/.cc/.cxx output)")
     a = ap.parse_args(argv)
+    out = a.output or ("svsim_circuit.cpp" if a.cplus else "svsim_circuit.py")
+    cplus = a.cplus or out.endswith((".cpp", ".cc", ".cxx"))
     n, gates, info = load(a.input)
-    with open(a.output, "w") as f:
-        f.write(to_script(gates, f"translated from {a.input}: {n} qubits"))
-    print(f"== SV-Sim: Translating {a.input} to {a.output} ==")
+    text = f"translated from {a.input}: {n} qubits"
+    with open(out, "w") as f:
+        f.write(to_cplus(n, gates, text) if cplus else to_script(gates, text))
+    print(f"== SV-Sim: Translating {a.input} to {out} ==")
     print(f"Number of qubits: {n}")
     print(f"Number of basic gates: {info['basic_gates']}")
     print(f"Number of cnot gates: {info['cx_gates']}")

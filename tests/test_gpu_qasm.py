@@ -84,3 +84,30 @@ def test_translator_script_and_append_qasm_through_libsvsim(tmp_path):
     n, g, _ = qasm.load(src)
     ore, oim = O.run(n, g)
     assert max(np.abs(re2 - ore).max(), np.abs(im2 - oim).max()) < TOL
+
+
+def test_cplus_translator_program_runs(tmp_path):
+    """The C++ program the translator writes (svsim_qasm_cplus.py's main:
+    upload, sim, measure(5), print_measurement) built against the facade and
+    run: adder_n10 ends in one basis state, so all 5 measured states are the
+    oracle's."""
+    n, g, _ = qasm.load(os.path.join(QDIR, "adder_n10.qasm"))
+    ore, oim = O.run(n, g)
+    p = ore ** 2 + oim ** 2
+    want = int(np.argmax(p))
+    assert abs(p[want] - 1) < 1e-12
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pkg = os.path.join(root, "sv-sim_amd")
+    cpp, exe = tmp_path / "adder.cpp", tmp_path / "adder_bin"
+    r = subprocess.run([sys.executable, "-m", "svmi.qasm", "-i", os.path.join(QDIR, "adder_n10.qasm"), "-o", str(cpp)],
+                       cwd=pkg, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run(["g++", "-std=c++17", "-O1", str(cpp), "-I", os.path.join(root, "include"),
+                        "-I", os.path.join(pkg, "csrc"), "-L", pkg, "-lsvmi", "-Wl,-rpath," + pkg, "-o", str(exe)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    print("case: adder_n10 C++ program", flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=90, env=dict(os.environ, SVSIM_QUIET="1"))
+    assert r.returncode == 0, r.stderr
+    tests = [ln.split(": ")[1] for ln in r.stdout.splitlines() if ln.startswith("Test-")]
+    assert "Measurement (tests=5)" in r.stdout and tests == [format(want, f"0{n}b")] * 5

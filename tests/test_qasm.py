@@ -158,3 +158,47 @@ def test_translator_command_writes_a_libsvsim_script(tmp_path):
     src = out.read_text()
     assert "sim.append(sim.U3(1.91063, 0.0, 0.0, 0))" in src and "sim.append(sim.CCX(0, 1, 2))" in src
     compile(src, str(out), "exec")
+
+
+ALL_GATES_QASM = """OPENQASM 2.0;
+include "qelib1.inc";
+qreg q[5];
+u3(0.1,0.2,0.3) q[0]; u2(0.4,0.5) q[1]; u1(0.6) q[2]; cx q[0],q[1]; id q[3]; x q[4]; y q[0]; z q[1];
+h q[2]; s q[3]; sdg q[4]; t q[0]; tdg q[1]; rx(0.7) q[2]; ry(0.8) q[3]; rz(0.9) q[4]; cz q[0],q[2];
+cy q[1],q[3]; swap q[2],q[4]; ch q[3],q[0]; ccx q[0],q[1],q[2]; cswap q[1],q[2],q[3]; crx(1.1) q[2],q[3];
+cry(1.2) q[3],q[4]; crz(1.3) q[4],q[0]; cu1(1.4) q[0],q[1]; cu3(1.5,1.6,1.7) q[1],q[2]; rxx(1.8) q[2],q[3];
+rzz(1.9) q[3],q[4]; rccx q[0],q[1],q[2]; rc3x q[0],q[1],q[2],q[3]; c3x q[1],q[2],q[3],q[4];
+c3sqrtx q[0],q[2],q[3],q[4]; c4x q[0],q[1],q[2],q[3],q[4];
+"""
+
+
+def test_translator_writes_a_cplus_program_that_builds(tmp_path):
+    """The C++ output (svsim/tool/svsim_qasm_cplus.py's program, on the
+    SVSim::Simulation facade): every append maps back to the parsed record,
+    and the program compiles and links against the library."""
+    src = tmp_path / "all.qasm"
+    src.write_text(ALL_GATES_QASM)
+    out = tmp_path / "all.cpp"
+    pkg = os.path.join(os.path.dirname(GOLD), "..", "sv-sim_amd")
+    r = subprocess.run([sys.executable, "-m", "svmi.qasm", "-i", str(src), "-o", str(out)],
+                       capture_output=True, text=True, cwd=pkg)
+    assert r.returncode == 0, r.stderr
+    assert "Number of qubits: 5" in r.stdout
+    n, gates, _ = qasm.load(str(src))
+    assert len({svmi.OP_NAMES[int(g["op"])] for g in gates}) >= 30
+    lines = [ln.strip() for ln in out.read_text().splitlines() if ln.strip().startswith("add(Simulation::")]
+    assert len(lines) == len(gates)
+    for ln, g in zip(lines, gates):
+        name, rest = ln[len("add(Simulation::"):].split("(", 1)
+        args = [a.strip() for a in rest.rsplit("));", 1)[0].split(",")]
+        assert name == svmi.OP_NAMES[int(g["op"])]
+        angles = qasm._ANGLES.get(name, ())
+        assert [float(a) for a in args[:len(angles)]] == [float(g[a]) for a in angles]
+        assert [int(a) for a in args[len(angles):]] == [int(q) for q in g["qb"][:qasm._ARITY.get(name, 1)]]
+    assert "int n_qubits = 5;" in out.read_text()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "all_bin"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", str(out), "-I", os.path.join(root, "include"),
+                        "-I", os.path.join(root, "sv-sim_amd", "csrc"), "-L", os.path.join(root, "sv-sim_amd"),
+                        "-lsvmi", "-o", str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
