@@ -23,6 +23,11 @@
 
 #include "svmi.h"
 
+// config.hpp:44: the seed applications pass to srand.
+#ifndef RAND_SEED
+#define RAND_SEED ((unsigned)time(0))
+#endif
+
 namespace SVSim {
 
 using IdxType = unsigned long long;
