@@ -116,6 +116,19 @@ SVMI_HD inline void dispatch_srn(int s, c64 (&a)[N])
     }
 }
 
+template <bool kThree, int N>
+SVMI_HD inline void dispatch_lift(int s, c64 (&a)[N], const c64* __restrict__ uv)
+{
+    const double u = uv->re, l = uv->im;
+    switch (s) {
+        SVMI_CASE_ID(0, 300 + 10 * kThree, (apply_lift<0, kThree, N>(a, u, l)))
+        SVMI_CASE_ID(1, 301 + 10 * kThree, (apply_lift<1, kThree, N>(a, u, l)))
+        SVMI_CASE_ID(2, 302 + 10 * kThree, (apply_lift<2, kThree, N>(a, u, l)))
+        SVMI_CASE_ID(3, 303 + 10 * kThree, (apply_lift<3, kThree, N>(a, u, l)))
+    default: break;
+    }
+}
+
 template <int N>
 SVMI_HD inline void dispatch_cdense(const DOpHdr& h, c64 (&a)[N], const c64* __restrict__ coef, uint32_t goff,
                                     uint64_t base)
@@ -250,6 +263,9 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag<N>(*op, a, coef, goff, off, base))
         SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset<N>(h, a, coef, goff, base, xf))
         SVMI_CASE_ID(DOP_CDENSE, DOP_CDENSE + 100, dispatch_cdense<N>(h, a, coef, goff, base))
+        SVMI_CASE_ID(DOP_LIFT2, DOP_LIFT2 + 100, (dispatch_lift<false, N>(h.s0, a, coef + h.coef)))
+        SVMI_CASE_ID(DOP_LIFT3, DOP_LIFT3 + 100, (dispatch_lift<true, N>(h.s0, a, coef + h.coef)))
+        SVMI_CASE_ID(DOP_RTAB, DOP_RTAB + 100, apply_rtab<N>(a, coef + h.coef + 16u * rtab_variant(h.emask, base)))
     default: break;
     }
 }

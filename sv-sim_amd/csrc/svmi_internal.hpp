@@ -97,7 +97,18 @@ enum DOpType : uint16_t {
     DOP_CDENSE = 11,     // 2x2 coef on register slot s0, on the register pairs in rsel
                          // where the group bits gmask and the bits emask outside the
                          // tile are all 1 (multi-controlled single-qubit gate)
+    // A real plane rotation on register slot s0 as in-place lifting steps
+    // (a0/a1 = the amplitudes with slot bit 0/1, each component separately):
+    DOP_LIFT2 = 12,      // a0 += u*a1; a1 += l*a0.  coef[0] = {u, l}.  With u = -s/c,
+                         // l = s*c this is diag(1/c, c) . [[c, -s], [s, c]]; the planner
+                         // folds the diag(c, 1/c) that completes the rotation into a table
+    DOP_LIFT3 = 13,      // a0 += u*a1; a1 += l*a0; a0 += u*a1 (three shears, no scale):
+                         // [[c, -s], [s, c]] with u = (c - 1)/s, l = s
+    DOP_RTAB = 14,       // diagonal over the register group: a[r] *= T[r], T = coef + 16*v,
+                         // v = the tile base's bits at the positions of emask (ascending),
+                         // so a table may depend on up to kMaxTabExt qubits outside the tile
 };
+constexpr int kMaxTabExt = 4;          // qubits outside the tile one register table may depend on
 
 // One controlled phase of a DOP_PHASESET (2 c64 slots in the coefficient
 // array): phase `ph` applies where the group bits gmask and the bits emask
@@ -312,6 +323,43 @@ SVMI_HD inline void apply_cdense(c64 (&a)[N], const c64* __restrict__ M, uint32_
             a[r] = cmac(cmac(c64{0.0, 0.0}, m00, a0), m01, a1);
             a[r | (1 << s)] = cmac(cmac(c64{0.0, 0.0}, m10, a0), m11, a1);
         }
+}
+
+// Lifting steps on slot s (DOP_LIFT2 / DOP_LIFT3).  Every step writes the
+// amplitude it updates in place, so an op leaves each value in the register
+// it came from (no copies where the op dispatch's cases join).
+template <int s, bool kThree, int N = kRegs>
+SVMI_HD inline void apply_lift(c64 (&a)[N], double u, double l)
+{
+#pragma unroll
+    for (int r = 0; r < N; r++)
+        if (!((r >> s) & 1)) {
+            c64& a0 = a[r];
+            c64& a1 = a[r | (1 << s)];
+            a0.re = SVMI_FMA(u, a1.re, a0.re);
+            a0.im = SVMI_FMA(u, a1.im, a0.im);
+            a1.re = SVMI_FMA(l, a0.re, a1.re);
+            a1.im = SVMI_FMA(l, a0.im, a1.im);
+            if constexpr (kThree) {
+                a0.re = SVMI_FMA(u, a1.re, a0.re);
+                a0.im = SVMI_FMA(u, a1.im, a0.im);
+            }
+        }
+}
+
+// Register table (DOP_RTAB): table variant of the tile at `base`.
+SVMI_HD inline uint32_t rtab_variant(uint64_t emask, uint64_t base)
+{
+    uint32_t v = 0, b = 0;
+    for (uint64_t m = emask; m; m &= m - 1, b++) v |= uint32_t((base >> __builtin_ctzll(m)) & 1u) << b;
+    return v;
+}
+
+template <int N = kRegs>
+SVMI_HD inline void apply_rtab(c64 (&a)[N], const c64* __restrict__ T)
+{
+#pragma unroll
+    for (int r = 0; r < N; r++) a[r] = cmul(a[r], T[r & (kRegs - 1)]);
 }
 
 // Reference SRN_GATE (svsim_cpu_omp.hpp:866-879), as coded there.

@@ -56,6 +56,14 @@ void emit_state_op(std::ostringstream& o, const DOp& d, const PhaseInfo& ph)
     case DOP_CX: o << "        apply_cx<" << h.s0 << ", " << h.s1 << ">(a);\n"; return;
     case DOP_SWAP: o << "        apply_swap<" << h.s0 << ", " << h.s1 << ">(a);\n"; return;
     case DOP_SRN: o << "        apply_srn<" << h.s0 << ">(a);\n"; return;
+    case DOP_LIFT2:
+    case DOP_LIFT3:
+        o << "        apply_lift<" << h.s0 << ", " << (h.type == DOP_LIFT3 ? "true" : "false") << ">(a, coef["
+          << h.coef << "].re, coef[" << h.coef << "].im);\n";
+        return;
+    case DOP_RTAB:
+        o << "        apply_rtab(a, coef + " << h.coef << "u + 16u * rtab_variant(" << h.emask << "ull, base));\n";
+        return;
     case DOP_CDENSE:
         o << "        if (" << cond << ") apply_cdense<" << h.s0 << ">(a, coef + " << h.coef << ", " << h.rsel
           << "u);\n";

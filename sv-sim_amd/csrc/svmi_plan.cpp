@@ -12,6 +12,7 @@
 #include "svmi_plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <deque>
 #include <cstring>
 #include <sstream>
@@ -343,10 +344,13 @@ struct Encoder {
 bool is_register_perm(const LOp& op);
 
 // X, CX and SWAP run as register moves; other dense ops as matrix products.
-long reg_cost(const LOp& op)
+// rot: the phase may run one-qubit ops as lifting rotations between merged
+// register tables (about 3 per amplitude, a diagonal about 1).
+long reg_cost(const LOp& op, bool rot)
 {
-    if (op.kind == LOp::DIAG) return 2;
+    if (op.kind == LOp::DIAG) return rot ? 1 : 2;
     if (is_register_perm(op)) return 1;
+    if (rot && op.k == 1) return 3;
     return 4L << op.k;
 }
 
@@ -408,6 +412,7 @@ struct RunMerger {
     std::deque<SOp>& sowned;
     int max_q;                 // widest merged op
     long overhead;             // per-op cost on top of its FMAs
+    bool rot;                  // cost one-qubit ops as lifting rotations (reg_cost)
     std::vector<const SOp*> run;
     uint64_t run_mask = 0;
 
@@ -433,7 +438,7 @@ struct RunMerger {
         long separate = 0;
         for (const SOp* s : run) {
             for (int i = 0; i < s->op->k; i++) S |= 1u << slot_of[tloc[s->op->q[i]]];
-            separate += reg_cost(*s->op) + overhead;
+            separate += reg_cost(*s->op, rot) + overhead;
         }
         const int q = __builtin_popcount(S);
         if (q > max_q || (4L << q) + overhead > separate) {
@@ -485,6 +490,255 @@ struct RunMerger {
         out.push_back(s);
     }
 };
+
+// ---------------------------------------------------------------------------
+// Rotation/table form of a one-state phase.  A one-qubit unitary U on a
+// register slot is D_a . [[c, -s], [s, c]] . D_b with D_a, D_b diagonal (the
+// ZYZ form, c, s >= 0); the real rotation runs as in-place lifting steps
+// (DOP_LIFT2 when c >= s, with its scale diag(c, 1/c) moved into D_a;
+// DOP_LIFT3 otherwise) and every diagonal factor -- the D parts, diagonal gates
+// on register slots and on qubits outside the tile -- joins the register table
+// (DOP_RTAB) of its run of commuting diagonals.  The phase then alternates
+// "one table" and "every rotation now free of pending diagonals".  Ops kept as
+// they are (permutations, wider dense blocks, diagonals on group bits,
+// controlled and collapse ops, SRN) are scheduled between them in dependency
+// order.  The caller keeps this form only when its cost is lower.
+// ---------------------------------------------------------------------------
+
+// Cost (FMA per amplitude + overhead per op) of an op kept as it is.
+long kept_cost(const SOp& s, long overhead)
+{
+    const LOp& op = *s.op;
+    if (op.kind == LOp::DIAG || op.kind == LOp::SRN) return 4 + overhead;
+    if (is_register_perm(op)) return 1 + overhead;
+    return (4L << op.k) + (op.k == 1 ? 2 : 0) + overhead;   // 2x2: plus the copies at the join
+}
+
+struct PItem {
+    enum Kind { OPAQUE = 0, DIAG = 1, LIFT = 2 } kind = OPAQUE;
+    const SOp* s = nullptr;   // OPAQUE, and DIAG items taken from an op
+    uint64_t mask = 0;        // physical qubits the item acts on
+    uint64_t dep = 0;         // physical qubits it must stay ordered with
+    bool diag = false;        // commutes with every other diagonal item
+    int q = -1;               // DIAG item of the decomposition: qubit q, entries d
+    c64 d[2];
+    int slot = 0;             // LIFT
+    bool three = false;
+    double u = 0.0, l = 0.0;
+};
+
+bool items_conflict(const PItem& x, const PItem& y)
+{
+    if (x.diag && y.diag) return false;
+    return (x.dep & y.dep) != 0;
+}
+
+// U (row-major 2x2) = diag(pa0, pa1) . [[c, -s], [s, c]] . diag(1, pb1) with
+// c, s >= 0 and unit pa0, pa1, pb1; false if U is diagonal or not of that form.
+bool zyz_split(const c64* U, c64& pa0, c64& pa1, c64& pb1, double& c, double& s)
+{
+    const double c0 = std::hypot(U[0].re, U[0].im), s0 = std::hypot(U[2].re, U[2].im);
+    const double nrm = std::hypot(c0, s0);
+    if (!(s0 > 0.0) || !(nrm > 0.5 && nrm < 2.0)) return false;
+    c = c0 / nrm;
+    s = s0 / nrm;
+    pa0 = c0 > 0.0 ? c64{U[0].re / c0, U[0].im / c0} : c64{1.0, 0.0};
+    pa1 = c64{U[2].re / s0, U[2].im / s0};
+    // -U01 = pa0 s pb1
+    const c64 m01{-U[1].re / s, -U[1].im / s};
+    pb1 = c64{pa0.re * m01.re + pa0.im * m01.im, pa0.re * m01.im - pa0.im * m01.re};
+    const double pn = std::hypot(pb1.re, pb1.im);
+    if (!(pn > 0.5 && pn < 2.0)) return false;
+    pb1 = c64{pb1.re / pn, pb1.im / pn};
+    // the four entries back from the factors
+    const c64 w[4] = {c64{pa0.re * c, pa0.im * c}, cmul(c64{-pa0.re * s, -pa0.im * s}, pb1),
+                      c64{pa1.re * s, pa1.im * s}, cmul(c64{pa1.re * c, pa1.im * c}, pb1)};
+    for (int i = 0; i < 4; i++)
+        if (std::fabs(w[i].re - U[i].re) > 1e-13 || std::fabs(w[i].im - U[i].im) > 1e-13) return false;
+    return true;
+}
+
+// One emitted op of a compiled phase: an op encoded as usual (s), or a
+// ready-made LIFT / RTAB op with its coefficients.
+struct PEmit {
+    const SOp* s = nullptr;
+    DOp d;
+    std::vector<c64> coef;
+};
+
+// Compile the phase's op list; returns the cost of the result.
+long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const int* slot_of, long overhead,
+                   std::vector<PEmit>& out)
+{
+    out.clear();
+    std::vector<PItem> items;
+    items.reserve(ops.size() * 3);
+    auto reg_slot = [&](int p) { return tloc[p] >= 0 ? slot_of[tloc[p]] : -1; };
+    for (const SOp* s : ops) {
+        const LOp& op = *s->op;
+        PItem it;
+        it.s = s;
+        it.mask = s->mask;
+        it.dep = s->dep;
+        it.diag = s->diag;
+        const bool plain = !s->grad && op.ctrl == 0 && op.dyn < 0;
+        if (plain && op.kind == LOp::DIAG) {
+            bool ok = true;   // every qubit a register slot or outside the tile
+            for (int i = 0; i < op.k; i++)
+                if (tloc[op.q[i]] >= 0 && reg_slot(op.q[i]) < 0) ok = false;
+            if (ok) it.kind = PItem::DIAG;
+            items.push_back(it);
+            continue;
+        }
+        c64 pa0, pa1, pb1;
+        double c = 0.0, sn = 0.0;
+        if (plain && op.kind == LOp::DENSE && op.k == 1 && reg_slot(op.q[0]) >= 0 && !is_register_perm(op) &&
+            zyz_split(op.m.data(), pa0, pa1, pb1, c, sn)) {
+            const uint64_t qm = uint64_t(1) << op.q[0];
+            PItem db;
+            db.kind = PItem::DIAG;
+            db.mask = db.dep = qm;
+            db.diag = true;
+            db.q = op.q[0];
+            db.d[0] = c64{1.0, 0.0};
+            db.d[1] = pb1;
+            PItem lf;
+            lf.kind = PItem::LIFT;
+            lf.mask = lf.dep = qm;
+            lf.slot = reg_slot(op.q[0]);
+            PItem da = db;
+            if (c >= sn) {
+                lf.u = -sn / c;
+                lf.l = sn * c;
+                da.d[0] = c64{pa0.re * c, pa0.im * c};
+                da.d[1] = c64{pa1.re / c, pa1.im / c};
+            } else {
+                lf.three = true;
+                lf.u = (c - 1.0) / sn;
+                lf.l = sn;
+                da.d[0] = pa0;
+                da.d[1] = pa1;
+            }
+            items.push_back(db);
+            items.push_back(lf);
+            items.push_back(da);
+            continue;
+        }
+        items.push_back(it);
+    }
+
+    long cost = 0;
+    std::vector<char> done(items.size(), 0);
+    size_t left = items.size();
+    auto ready = [&](size_t i) {
+        for (size_t j = 0; j < i; j++)
+            if (!done[j] && items_conflict(items[j], items[i])) return false;
+        return true;
+    };
+    auto emit_op = [&](const SOp* s) {
+        PEmit e;
+        e.s = s;
+        out.push_back(std::move(e));
+        cost += kept_cost(*s, overhead);
+    };
+    while (left) {
+        const size_t left0 = left;
+        // diagonals free of pending non-diagonal items: one table (+ ops kept)
+        std::vector<size_t> tab;
+        std::vector<const SOp*> kept;
+        uint64_t emask = 0;
+        for (size_t i = 0; i < items.size(); i++) {
+            if (done[i] || !items[i].diag || !ready(i)) continue;
+            done[i] = 1;
+            left--;
+            const PItem& it = items[i];
+            if (it.kind != PItem::DIAG) {
+                kept.push_back(it.s);
+                continue;
+            }
+            uint64_t ext = 0;
+            if (it.s)
+                for (int j = 0; j < it.s->op->k; j++)
+                    if (tloc[it.s->op->q[j]] < 0) ext |= uint64_t(1) << it.s->op->q[j];
+            if (__builtin_popcountll(emask | ext) > kMaxTabExt) {
+                kept.push_back(it.s);
+                continue;
+            }
+            emask |= ext;
+            tab.push_back(i);
+        }
+        if (!tab.empty()) {
+            const int ne = __builtin_popcountll(emask);
+            int epos[kMaxTabExt];
+            {
+                int j = 0;
+                for (uint64_t m = emask; m; m &= m - 1) epos[j++] = __builtin_ctzll(m);
+            }
+            std::vector<c64> T(size_t(kRegs) << ne, c64{1.0, 0.0});
+            for (size_t v = 0; v < (size_t(1) << ne); v++)
+                for (int r = 0; r < kRegs; r++) {
+                    c64 f{1.0, 0.0};
+                    for (size_t i : tab) {
+                        const PItem& it = items[i];
+                        if (!it.s) {
+                            f = cmul(f, it.d[(r >> reg_slot(it.q)) & 1]);
+                            continue;
+                        }
+                        const LOp& op = *it.s->op;
+                        size_t idx = 0;
+                        for (int j = 0; j < op.k; j++) {
+                            const int p = op.q[j];
+                            int bit;
+                            if (tloc[p] >= 0) {
+                                bit = (r >> reg_slot(p)) & 1;
+                            } else {
+                                int e = 0;
+                                while (epos[e] != p) e++;
+                                bit = int(v >> e) & 1;
+                            }
+                            idx |= size_t(bit) << j;
+                        }
+                        f = cmul(f, op.m[idx]);
+                    }
+                    T[v * kRegs + size_t(r)] = f;
+                }
+            bool ident = true;
+            for (const c64& x : T) ident = ident && x.re == 1.0 && x.im == 0.0;
+            if (!ident) {
+                PEmit e;
+                std::memset(&e.d, 0, sizeof(e.d));
+                e.d.targets = DOP_TO_PSI;
+                e.d.h.type = DOP_RTAB;
+                e.d.h.emask = emask;
+                e.coef.swap(T);
+                out.push_back(std::move(e));
+                cost += 4 + overhead;
+            }
+        }
+        for (const SOp* s : kept) emit_op(s);
+        // everything non-diagonal that no pending item precedes
+        for (size_t i = 0; i < items.size(); i++) {
+            if (done[i] || items[i].diag || !ready(i)) continue;
+            done[i] = 1;
+            left--;
+            const PItem& it = items[i];
+            if (it.kind == PItem::LIFT) {
+                PEmit e;
+                std::memset(&e.d, 0, sizeof(e.d));
+                e.d.targets = DOP_TO_PSI;
+                e.d.h.type = it.three ? DOP_LIFT3 : DOP_LIFT2;
+                e.d.h.s0 = uint32_t(it.slot);
+                e.coef.push_back(c64{it.u, it.l});
+                out.push_back(std::move(e));
+                cost += (it.three ? 3 : 2) + overhead;
+            } else {
+                emit_op(it.s);
+            }
+        }
+        if (left == left0) throw std::logic_error("compile_phase: no item ready");
+    }
+    return cost;
+}
 
 // Emit a standalone full-state op step.
 void emit_standalone(Program& prog, const SOp& s)
@@ -648,11 +902,45 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         std::vector<const SOp*> merged;
         {
             RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged, sowned,
-                         prog.n_states == 2 ? kMaxDenseQubitsAdj : kRegBits, opt.op_overhead, {}, 0};
+                         prog.n_states == 2 ? kMaxDenseQubitsAdj : kRegBits, opt.op_overhead, false, {}, 0};
             for (const SOp* s : ph) rm.add(s);
             rm.flush();
         }
         pi.op_begin = uint32_t(prog.ops.size());
+        if (opt.rotations && prog.n_states == 1) {
+            std::vector<const SOp*> merged_rot;
+            RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged_rot, sowned, kRegBits, opt.op_overhead, true, {},
+                         0};
+            for (const SOp* s : ph) rm.add(s);
+            rm.flush();
+            std::vector<PEmit> em;
+            const long c_rot = compile_phase(merged_rot, tloc, slot_of, opt.op_overhead, em);
+            long c_kept = 0;
+            for (const SOp* s : merged) c_kept += kept_cost(*s, opt.op_overhead);
+            if (c_rot < c_kept) {
+                for (size_t i = 0; i < em.size();) {
+                    if (!em[i].s) {
+                        DOp d = em[i].d;
+                        d.h.coef = enc.push_coef(em[i].coef);
+                        prog.ops.push_back(d);
+                        i++;
+                        continue;
+                    }
+                    size_t j = i;
+                    while (j < em.size() && em[j].s && is_cphase(*em[j].s)) j++;
+                    if (j - i >= 2) {
+                        std::vector<const SOp*> run;
+                        for (size_t t = i; t < j; t++) run.push_back(em[t].s);
+                        enc.encode_phasesets(run, 0, run.size(), tloc, slot_of);
+                        i = j;
+                        continue;
+                    }
+                    enc.encode(*em[i].s, tloc, slot_of, 0);
+                    i++;
+                }
+                merged.clear();
+            }
+        }
         for (size_t i = 0; i < merged.size();) {
             // a run of controlled-phase ops commutes; group it by register
             // condition into phase-product ops
@@ -901,13 +1189,15 @@ std::string describe_program(const Program& prog)
                 ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3] << " :";
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                     const DOp& d = prog.ops[o];
-                    static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS", "CD"};
+                    static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS", "CD",
+                                               "L", "L3", "T"};
                     ss << " " << nm[d.h.type];
                     if (d.h.type == DOP_PHASESET) {
                         ss << d.h.s0;
                         if (d.h.s1) ss << "u" << d.h.s1;
                     }
                     if (d.h.type == DOP_DENSE || d.h.type == DOP_GRAD_DENSE) ss << __builtin_popcount(d.h.smask);
+                    if (d.h.type == DOP_RTAB && d.h.emask) ss << "e" << __builtin_popcountll(d.h.emask);
                 }
                 ss << "\n";
             }

@@ -16,6 +16,8 @@ struct PlanOptions {
     long sweep_budget = 0;     // per-amplitude FMA budget of one sweep (0 = none)
     long op_overhead = 2;      // fusion cost model: per-op cost on top of its FMAs (per amplitude)
     long coef_budget = 20000;  // split a sweep whose op coefficients exceed this many bytes (0 = never)
+    bool rotations = true;     // one-state phases: one-qubit unitaries as lifting rotations between
+                               // merged register tables where the cost model prefers it
 };
 
 // One step of a program: a tiled sweep or a standalone full-state op.
