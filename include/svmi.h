@@ -250,7 +250,13 @@ svmi_status svmi_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates
  *      bit set and divides the rows with the outcome's bit by P(1) (or
  *      1 - P(1)) and zeroes the others, rows only (1710-1830).  The engine's
  *      noise superoperators are not applied there (their calls are commented
- *      out, 2792-2840), nor here.  probs / outcomes as above. */
+ *      out, 2792-2840), nor here.  probs / outcomes as above.  Control masks
+ *      follow that engine's OP_HEAD_MASK (619-628) over the 2 n_qubits-qubit
+ *      state: a control bit c in [n_qubits, 2 n_qubits) makes the row half
+ *      act where column qubit c - n_qubits is 1, and the column half (mask
+ *      shifted past the state) the identity -- rho then need not stay
+ *      Hermitian, exactly as in the reference; a bit >= 2 n_qubits or one
+ *      naming the target makes the whole op the identity. */
 svmi_status svmi_dm_qir_run(svmi_handle h, int n_qubits, const svmi_qir_gate* gates, size_t n_gates,
                             double* probs, int32_t* outcomes);
 

@@ -174,13 +174,13 @@ public:
         svmi_throw(svmi_clear_circuit(h), "clear_circuit");
     }
     // svsim_cpu_omp.hpp:339-361: draws r = rand()/RAND_MAX after
-    // srand(time(0)), the reference's RAND_SEED (config.hpp:44).
+    // srand(RAND_SEED) (config.hpp:44: time(0) unless the build defines it).
     // Returns new IdxType[repetition], owned by the caller (delete[]), as
     // the reference's measure returns its array.
     IdxType* measure(unsigned repetition = 10)
     {
         std::vector<double> u(repetition);
-        srand(unsigned(time(0)));
+        srand(unsigned(RAND_SEED));
         for (unsigned i = 0; i < repetition; i++) u[i] = double(rand()) / double(RAND_MAX);
         const std::vector<IdxType> r = measure_with(u);
         IdxType* res_state = new IdxType[repetition];

@@ -41,3 +41,19 @@ def test_limits():
         svmi.emulate_dm_qir(14, [(O.QIR["H"], 0, 0, 0.0)])
     with pytest.raises(svmi.SvmiError):
         svmi.emulate_dm_qir(2, [(O.QIR["Swap"], 0, 2, 0.0)])   # second qubit not allocated
+
+
+def test_control_bit_on_a_column_qubit():
+    """A control bit in [n, 2n) acts on the row half as a control on column
+    qubit c - n and leaves the column half alone (the reference engine's
+    OP_HEAD_MASK over the 2n-qubit state, svsim_qsharp_noise/svsim_cpu_omp.hpp:
+    619-628): the result matches the oracle and is not Hermitian."""
+    n = 3
+    g = [(O.QIR["H"], 0, 0, 0.0), (O.QIR["H"], 1, 0, 0.0),
+         (O.QIR["ControlledX"], 2, 1 << n, 0.0),            # control: column qubit 0
+         (O.QIR["ControlledRY"], 1, (1 << n) | 1, 0.7)]     # controls: row 0 and column 0
+    re, im, _, _ = svmi.emulate_dm_qir(n, g)
+    ore, oim, _, _ = O.dm_qir_apply(n, g)
+    assert max(np.abs(re - ore).max(), np.abs(im - oim).max()) < 1e-13
+    rho = (ore + 1j * oim).reshape(1 << n, 1 << n).T       # rho[row, col], index row | col << n
+    assert np.abs(rho - rho.conj().T).max() > 0.1
