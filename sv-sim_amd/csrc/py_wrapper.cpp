@@ -1,6 +1,8 @@
 // py_wrapper.cpp -- pybind11 module `libsvsim` with the reference's Python
 // surface (svsim/src/py_wrapper.cpp:53-109): Gate, Simulation(n_qubits,
-// n_dev), append/upload/clear_circuit/run/reset/measure and the 38 static
+// i_gpu) -- the second argument is the HIP ordinal, as in the AMD single-GPU
+// build of that module (svsim_amdgpu_sin.hpp:207-220); an ordinal that is not
+// present raises --, append/upload/clear_circuit/run/reset/measure and the 38 static
 // gate factories.  Additions (no reference counterpart): the OP enum,
 // get_sv/set_sv, measure_with(draws), expval, adjoint, last_sim_ms.
 //

@@ -813,7 +813,8 @@ svmi_status svmi_create(int n_qubits, int device, svmi_handle* out)
     SVMI_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev)
         return fail(SVMI_ERR_DEVICE, "device " + std::to_string(device) + " not present (" + std::to_string(ndev) +
-                                         " visible)");
+                                         " visible): the second argument is the HIP ordinal of the one GPU the "
+                                         "engine runs on (the single-GPU engine's i_gpu), not a device or core count");
     SVMI_HIP(hipSetDevice(device));
     SVMI_TRY(ensure_kernels());
     svmi_sim* h = new svmi_sim();

@@ -509,6 +509,7 @@ struct RunMerger {
 long kept_cost(const SOp& s, long overhead)
 {
     const LOp& op = *s.op;
+    if (s.grad) return (op.kind == LOp::DIAG ? 6 : (8L << op.k)) + overhead;
     if (op.kind == LOp::DIAG || op.kind == LOp::SRN) return 4 + overhead;
     if (is_register_perm(op)) return 1 + overhead;
     return (4L << op.k) + (op.k == 1 ? 2 : 0) + overhead;   // 2x2: plus the copies at the join
@@ -566,9 +567,16 @@ struct PEmit {
     std::vector<c64> coef;
 };
 
-// Compile the phase's op list; returns the cost of the result.
+// Compile the phase's op list; returns the cost of the result.  two_state:
+// an adjoint phase (every state op acts on phi and lambda, gradient ops read
+// both).  Its rotations all use three shears: DOP_LIFT2 leaves the state
+// scaled by a real diagonal until the table that follows, and a gradient op
+// scheduled in between would read the scaled amplitudes.  Every other item
+// keeps the pair orthogonally or unitarily equivalent, so a gradient op
+// (diagonal K) commutes with every diagonal item and (dense K) with every
+// item on other qubits.
 long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const int* slot_of, long overhead,
-                   std::vector<PEmit>& out)
+                   bool two_state, std::vector<PEmit>& out)
 {
     out.clear();
     std::vector<PItem> items;
@@ -607,7 +615,7 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
             lf.mask = lf.dep = qm;
             lf.slot = reg_slot(op.q[0]);
             PItem da = db;
-            if (c >= sn) {
+            if (c >= sn && !two_state) {
                 lf.u = -sn / c;
                 lf.l = sn * c;
                 da.d[0] = c64{pa0.re * c, pa0.im * c};
@@ -707,7 +715,7 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
             if (!ident) {
                 PEmit e;
                 std::memset(&e.d, 0, sizeof(e.d));
-                e.d.targets = DOP_TO_PSI;
+                e.d.targets = two_state ? DOP_TO_BOTH : DOP_TO_PSI;
                 e.d.h.type = DOP_RTAB;
                 e.d.h.emask = emask;
                 e.coef.swap(T);
@@ -725,7 +733,7 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
             if (it.kind == PItem::LIFT) {
                 PEmit e;
                 std::memset(&e.d, 0, sizeof(e.d));
-                e.d.targets = DOP_TO_PSI;
+                e.d.targets = two_state ? DOP_TO_BOTH : DOP_TO_PSI;
                 e.d.h.type = it.three ? DOP_LIFT3 : DOP_LIFT2;
                 e.d.h.s0 = uint32_t(it.slot);
                 e.coef.push_back(c64{it.u, it.l});
@@ -907,14 +915,15 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             rm.flush();
         }
         pi.op_begin = uint32_t(prog.ops.size());
-        if (opt.rotations && prog.n_states == 1) {
+        if (opt.rotations) {
+            const bool two = prog.n_states == 2;
             std::vector<const SOp*> merged_rot;
-            RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged_rot, sowned, kRegBits, opt.op_overhead, true, {},
-                         0};
+            RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged_rot, sowned,
+                         two ? kMaxDenseQubitsAdj : kRegBits, opt.op_overhead, true, {}, 0};
             for (const SOp* s : ph) rm.add(s);
             rm.flush();
             std::vector<PEmit> em;
-            const long c_rot = compile_phase(merged_rot, tloc, slot_of, opt.op_overhead, em);
+            const long c_rot = compile_phase(merged_rot, tloc, slot_of, opt.op_overhead, two, em);
             long c_kept = 0;
             for (const SOp* s : merged) c_kept += kept_cost(*s, opt.op_overhead);
             if (c_rot < c_kept) {
@@ -927,7 +936,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                         continue;
                     }
                     size_t j = i;
-                    while (j < em.size() && em[j].s && is_cphase(*em[j].s)) j++;
+                    while (j < em.size() && em[j].s && is_cphase(*em[j].s) && !em[j].s->grad) j++;
                     if (j - i >= 2) {
                         std::vector<const SOp*> run;
                         for (size_t t = i; t < j; t++) run.push_back(em[t].s);
@@ -935,8 +944,13 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                         i = j;
                         continue;
                     }
-                    enc.encode(*em[i].s, tloc, slot_of, 0);
-                    i++;
+                    const SOp* s = em[i++].s;
+                    uint32_t gslot = 0;
+                    if (s->grad) {
+                        gslot = n_grad_sweep++;
+                        prog.grad_param.push_back(s->param);
+                    }
+                    enc.encode(*s, tloc, slot_of, gslot);
                 }
                 merged.clear();
             }

@@ -382,6 +382,23 @@ def test_errors_are_reported():
         svmi.Engine(4, device=64)
 
 
+def test_second_constructor_argument_is_the_device_ordinal():
+    """Simulation(n, i) selects HIP device i, as the AMD single-GPU engine's
+    constructor does (svsim/src/svsim_amdgpu_sin.hpp:207-220); an ordinal
+    past the visible devices (e.g. a core count meant for the CPU build) is
+    a clear error, never a silent device change."""
+    import libsvsim as svsim
+    import torch
+    ndev = torch.cuda.device_count()
+    sim = svsim.Simulation(3, 0)
+    sim.append(sim.X(1))
+    sim.upload()
+    sim.run()
+    assert sim.measure(3) == [2, 2, 2]
+    with pytest.raises(RuntimeError, match="HIP ordinal"):
+        svsim.Simulation(3, ndev)
+
+
 def test_libsvsim_dropin_adder():
     # svsim/example/adder_n10.py with the reference API, then the measured
     # basis state is checked: 1 + 15 on the Cuccaro adder -> deterministic
