@@ -41,6 +41,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """Cores this process may run on (the CPU baselines use all of them)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def bench_u3_brickwall(n, steps, warmup):
     import svmi
     from svmi import workloads as W
@@ -179,26 +198,175 @@ def bench_adjoint(n=20, layers=8, reps=5):
             "max_abs_diff_vs_param_shift": float(np.abs(g - ps).max())}
 
 
-def bench_batch(count=512):
+def bench_batch(count=512, reps=3):
     """Expectation + adjoint gradient of many uneven ansatz circuits: one
-    batched call against a per-circuit loop on the same GPU."""
+    batched call against a per-circuit loop on the same GPU.  One untimed
+    warmup of each, then the median of `reps` timed runs of each."""
     import svmi
     from svmi import workloads as W
     items = W.batch_ising(count=count)
-    t0 = time.perf_counter()
+
+    def loop():
+        ev2 = np.zeros(len(items))
+        gs = []
+        for i, it in enumerate(items):
+            with svmi.Engine(it["n"]) as e:
+                e.upload(it["gates"])
+                ev2[i], g = e.adjoint(it["terms"], it["param_slots"], it["n_params"])
+                gs.append(g)
+        return ev2, gs
+
     ev, grads = svmi.batch_adjoint(items)
-    t_batch = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    ev2 = np.zeros(len(items))
-    diff = 0.0
-    for i, it in enumerate(items):
-        with svmi.Engine(it["n"]) as e:
-            e.upload(it["gates"])
-            ev2[i], g = e.adjoint(it["terms"], it["param_slots"], it["n_params"])
-            diff = max(diff, float(np.abs(g - grads[i]).max()))
-    t_loop = time.perf_counter() - t0
+    ev2, g2 = loop()
+    tb, tl = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ev, grads = svmi.batch_adjoint(items)
+        tb.append(time.perf_counter() - t0)
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ev2, g2 = loop()
+        tl.append(time.perf_counter() - t0)
+    diff = max(float(np.abs(a - b).max()) for a, b in zip(grads, g2))
+    t_batch, t_loop = float(np.median(tb)), float(np.median(tl))
+    # share of the loop's device work in the largest circuits (work ~ gates * 2^n)
+    w = np.array([len(it["gates"]) * float(1 << it["n"]) for it in items])
+    big = np.array([it["n"] >= 19 for it in items])
     return {"circuits": len(items), "batched_ms": t_batch * 1e3, "loop_ms": t_loop * 1e3,
-            "speedup": t_loop / t_batch, "max_abs_diff": max(diff, float(np.abs(ev - ev2).max()))}
+            "batched_runs_ms": [t * 1e3 for t in tb], "loop_runs_ms": [t * 1e3 for t in tl],
+            "speedup": t_loop / t_batch, "max_abs_diff": max(diff, float(np.abs(ev - ev2).max())),
+            "work_share_n_ge_19": float(w[big].sum() / w.sum())}
+
+
+def bench_c1(reps=10):
+    """The 12-qubit random-layer circuit (720 gates) from |0..0> plus 1024
+    shots: GPU wall time per run (median of reps, each from a reset state,
+    synchronised), and the reference CPU engine (oracle/_ref, compiled from
+    svsim/src/svsim_cpu_omp.hpp) timing only its sim() as the reference does,
+    median of 10 runs on every host core; the oracle restatement when the
+    reference build is absent."""
+    import svmi
+    from oracle import oracle as O
+    from svmi import workloads as W
+    n = 12
+    gates = W.random_layers(n=n, layers=40)
+    _, u = W.draws(1024, seed=2001)
+    e = svmi.Engine(n)
+    e.upload(gates)
+    e.run()
+    runs, shots_t = [], []
+    for _ in range(reps):
+        e.reset_state()
+        t0 = time.perf_counter()
+        e.run()
+        runs.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        shots, _ = e.measure(u)
+        shots_t.append(time.perf_counter() - t0)
+    re, im = e.get_state()
+    dev = e.stats()["device_ms"]
+    e.close()
+    t = float(np.median(runs))
+    out = {"qubits": n, "gates": len(gates), "gpu_run_ms": t * 1e3, "gpu_device_ms": dev,
+           "gpu_gates_per_s": len(gates) / t, "gpu_1024_shots_ms": float(np.median(shots_t)) * 1e3}
+    cores = host_cores()
+    if O.ref_available():
+        ms = O.ref_sim_times(n, gates, 10, nthreads=cores)
+        ore, oim = O.ref_run(n, gates, nthreads=cores)
+        kind = "reference"
+    else:
+        ms = []
+        for _ in range(10):
+            t0 = time.perf_counter()
+            ore, oim = O.run(n, gates, nthreads=cores)
+            ms.append((time.perf_counter() - t0) * 1e3)
+        kind = "port"
+    cpu_ms = float(np.median(ms))
+    out["cpu"] = {"value": len(gates) / (cpu_ms * 1e-3), "unit": "gates/s", "cores": cores, "kind": kind,
+                  "sim_ms_median_of_10": cpu_ms, "cpu_model": host_cpu_model(),
+                  "sample": "whole circuit, sim() only, median of 10 runs"}
+    out["max_abs_diff_vs_cpu"] = float(max(np.abs(re - ore).max(), np.abs(im - oim).max()))
+    out["shots_equal_oracle"] = bool(np.array_equal(shots, O.measure(n, ore, oim, u)))
+    return out
+
+
+def cpu_param_shift_c4(budget_s=15.0, threads=None):
+    """CPU parameter shift (oracle, OpenMP) for the 20-qubit 320-parameter
+    ansatz: each parameter needs two forward passes + <H>; a bounded number
+    of parameters is timed and the full gradient's time extrapolated from
+    them (all 640 passes would take minutes)."""
+    from oracle import oracle as O
+    from svmi import workloads as W
+    threads = threads or host_cores()
+    gates, slots, P, terms = W.hea_ising(n=20, layers=8)
+    names = ("theta", "phi", "lambda")
+    bound = [(gi, s_) for gi in range(len(gates)) for s_ in range(3) if slots[gi, s_] >= 0]
+    done, t = 0, 0.0
+    for gi, s_ in bound:
+        if t >= budget_s:
+            break
+        t0 = time.perf_counter()
+        for sh in (np.pi / 2, -np.pi / 2):
+            g = gates.copy()
+            g[gi][names[s_]] += sh
+            re, im = O.run(20, g, nthreads=threads)
+            O.expval(20, re, im, terms, nthreads=threads)
+        t += time.perf_counter() - t0
+        done += 1
+    per = t / done
+    return {"value": per * len(bound) * 1e3, "unit": "ms (full 320-parameter gradient, extrapolated)",
+            "cores": threads, "kind": "port", "cpu_model": host_cpu_model(),
+            "sample": f"{done} of {len(bound)} parameters ({2 * done} forward passes + <H>), {t:.1f} s"}
+
+
+def c5_cpu_subset(threads=None, reps_grad=2):
+    """16 circuits of the 512-circuit batch spanning n = 10..22: <H> on the
+    CPU (oracle) against the GPU batch, and the gradient entries of the first
+    `reps_grad` parameters of each by CPU parameter shift; CPU time per
+    circuit for <H> and, extrapolated from the shifted passes, for its whole
+    gradient."""
+    import svmi
+    from oracle import oracle as O
+    from svmi import workloads as W
+    threads = threads or host_cores()
+    items = W.batch_ising(count=512)
+    pick = []
+    for n in range(10, 23):
+        idx = [i for i, it in enumerate(items) if it["n"] == n]
+        if idx:
+            pick.append(idx[0])
+    for i, it in enumerate(items):
+        if len(pick) >= 16:
+            break
+        if i not in pick and it["n"] >= 20:
+            pick.append(i)
+    sub = [items[i] for i in sorted(pick)]
+    ev, grads = svmi.batch_adjoint(sub)
+    names = ("theta", "phi", "lambda")
+    t_ev, t_grad_est, dev, gdev = 0.0, 0.0, 0.0, 0.0
+    for k, it in enumerate(sub):
+        n = it["n"]
+        t0 = time.perf_counter()
+        re, im = O.run(n, it["gates"], nthreads=threads)
+        e_cpu = O.expval(n, re, im, it["terms"], nthreads=threads)
+        t1 = time.perf_counter() - t0
+        t_ev += t1
+        dev = max(dev, abs(e_cpu - ev[k]))
+        slots = np.asarray(it["param_slots"]).reshape(len(it["gates"]), 3)
+        bound = [(gi, s_) for gi in range(len(it["gates"])) for s_ in range(3) if slots[gi, s_] >= 0]
+        for gi, s_ in bound[:reps_grad]:
+            vals = []
+            for sh in (np.pi / 2, -np.pi / 2):
+                g = it["gates"].copy()
+                g[gi][names[s_]] += sh
+                r2, i2 = O.run(n, g, nthreads=threads)
+                vals.append(O.expval(n, r2, i2, it["terms"], nthreads=threads))
+            gdev = max(gdev, abs(0.5 * (vals[0] - vals[1]) - grads[k][slots[gi, s_]]))
+        t_grad_est += t1 * 2 * len(bound)
+    return {"circuits": len(sub), "qubits": sorted(int(it["n"]) for it in sub),
+            "cpu_expval_ms": t_ev * 1e3, "cpu_param_shift_ms_extrapolated": t_grad_est * 1e3,
+            "max_abs_diff_expval": float(dev), "max_abs_diff_grad_sampled": float(gdev),
+            "cores": threads, "kind": "port", "cpu_model": host_cpu_model()}
 
 
 def bench_qir(n=30, reps=3):
@@ -285,7 +453,7 @@ def cpu_baseline_dm(n=15, budget_s=20.0, threads=None):
     engine itself stops at 13 qubits: 2n < N_QUBIT_SLOT = 28.)"""
     from oracle import oracle as O
     from svmi import workloads as W
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_cores()
     g = W.qir_mixed(n=n, n_gates=600, n_measures=8)
     t0 = time.perf_counter()
     O.dm_qir_apply(n, g[:1], nthreads=threads)
@@ -336,7 +504,7 @@ def cpu_reference_qasm(name="qft_n20", threads=None):
     from svmi import qasm
     if not O.ref_available():
         return None
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_cores()
     n, gates, _ = qasm.load(os.path.join(ROOT, "tests", "golden", "qasm", name + ".qasm"))
     t0 = time.perf_counter()
     O.ref_run(n, gates, nthreads=threads)
@@ -350,7 +518,7 @@ def cpu_baseline_qir(n, budget_s=10.0, threads=None):
     OpenMP) on the first records of the same program, until ~budget_s."""
     from oracle import oracle as O
     from svmi import workloads as W
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_cores()
     g = W.qir_mixed(n=n)
     re, im = O.zero_state(n)
     done, t = 0, 0.0
@@ -417,7 +585,7 @@ def cpu_baseline_u3(n, budget_s=12.0, threads=None):
     gates of the same circuit, in chunks until ~budget_s of CPU work."""
     from oracle import oracle as O
     from svmi import workloads as W
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_cores()
     gates = W.u3_brickwall(n=n, layers=20)
     re, im = O.zero_state(n)
     done, t = 0, 0.0
@@ -429,8 +597,9 @@ def cpu_baseline_u3(n, budget_s=12.0, threads=None):
         if rc != 0:
             raise RuntimeError("oracle failed")
         done += len(chunk)
-    return {"value": done / t, "unit": "gates/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} of 890 gates of the {n}-qubit u3 brick-wall circuit, {t:.1f} s"}
+    return {"value": done / t, "unit": "gates/s", "cores": threads, "kind": "port", "cpu_model": host_cpu_model(),
+            "sample": f"first {done} of 890 gates of the {n}-qubit u3 brick-wall circuit, {t:.1f} s "
+                      "(the reference engine stops at 28 qubits: 32-bit IdxType)"}
 
 
 def main():
@@ -441,7 +610,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
-    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm", "specialized", "dm"),
+    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm", "specialized", "dm", "c1"),
                     help="run only this extra workload and print its result (profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (HBM counter calibration)")
@@ -458,7 +627,7 @@ def main():
         fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch,
               "qir": lambda: bench_qir(30, reps=a.steps), "qasm": lambda: bench_qasm(reps=a.steps),
               "specialized": lambda: bench_specialized(steps=a.steps),
-              "dm": lambda: bench_dm(15, reps=a.steps)}[a.only]
+              "dm": lambda: bench_dm(15, reps=a.steps), "c1": bench_c1}[a.only]
         log(f"[bench] only {a.only}")
         print(json.dumps({a.only: fn()}), flush=True)
         return
@@ -474,6 +643,9 @@ def main():
     log(f"[bench] {ms:.1f} ms/step, {r['passes']} passes/step, sweep avg {avg_sweep_s*1e3:.3f} ms, "
         f"{achieved:.0f} GB/s, sampling 4096 shots {r['sample_s']*1e3:.1f} ms")
     extra = {"stream_copy_GBps": copy_gbps}
+    log("[bench] C1: 12-qubit random layers + 1024 shots, GPU vs reference CPU engine")
+    extra["c1_random_layers_n12"] = bench_c1()
+    log(f"[bench] {extra['c1_random_layers_n12']}")
     if not a.quick:
         log("[bench] qft n=33 on a seeded random input")
         extra["qft_n33"] = bench_qft(33, steps=2)
@@ -482,9 +654,15 @@ def main():
         log("[bench] adjoint gradient, 20-qubit ansatz, 320 parameters")
         extra["adjoint_hea_n20"] = bench_adjoint()
         log(f"[bench] {extra['adjoint_hea_n20']}")
+        if not a.no_cpu:
+            extra["adjoint_hea_n20"]["cpu_param_shift"] = cpu_param_shift_c4()
+            log(f"[bench] C4 cpu {extra['adjoint_hea_n20']['cpu_param_shift']}")
         log("[bench] 512 batched ansatz gradients vs per-circuit loop")
         extra["batch_ising_512"] = bench_batch()
         log(f"[bench] {extra['batch_ising_512']}")
+        if not a.no_cpu:
+            extra["batch_ising_512"]["cpu_subset"] = c5_cpu_subset()
+            log(f"[bench] C5 cpu subset {extra['batch_ising_512']['cpu_subset']}")
         log("[bench] QIR gate-set program, 30 qubits, 16 mid-circuit measurements")
         extra["qir_mixed_n30"] = bench_qir(30)
         log(f"[bench] {extra['qir_mixed_n30']}")

@@ -103,6 +103,8 @@ def ref():
         L.ref_measure.argtypes = [ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t, _vp]
         L.ref_measure.restype = ctypes.c_int
         L.ref_rand_max.restype = ctypes.c_int
+        L.ref_sim_times.argtypes = [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_size_t, ctypes.c_int, _vp]
+        L.ref_sim_times.restype = ctypes.c_int
         _ref = L
     return _ref
 
@@ -234,6 +236,19 @@ def ref_run(n, gates, state=None, nthreads=1):
     if rc != 0:
         raise ValueError("reference driver rejected the circuit")
     return ore, oim
+
+
+def ref_sim_times(n, gates, reps, nthreads=1):
+    """Milliseconds of Simulation::sim() alone for `reps` runs from |0..0>
+    (what the reference itself times, svsim/src/svsim_cpu_omp.hpp:288-319)."""
+    L = ref()
+    if L is None:
+        raise RuntimeError("oracle/_ref not built (reference sources absent)")
+    g = as_gates(gates)
+    ms = np.zeros(int(reps))
+    if L.ref_sim_times(n, int(nthreads), _ptr(g), len(g), int(reps), _ptr(ms)) != 0:
+        raise ValueError("reference driver rejected the circuit")
+    return ms
 
 
 def ref_measure(n, re, im, draws):

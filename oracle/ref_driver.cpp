@@ -29,6 +29,8 @@ void srand(unsigned) { g_pos = 0; }
 }  // namespace SVSim
 
 #include "util_cpu.h"
+#include <chrono>
+
 #include "svsim_cpu_omp.hpp"
 
 using namespace SVSim;
@@ -124,6 +126,33 @@ int ref_measure(int n, const double* re, const double* im,
     delete[] res;
     g_draws = nullptr;
     g_ndraws = 0;
+    return 0;
+}
+
+// Wall time of Simulation::sim() alone (the part the reference times,
+// cpu_omp:288-319) for `reps` runs of the circuit, each from |0..0>
+// (reset_sv, cpu_omp:234-240): ms[r] in milliseconds.
+int ref_sim_times(int n, int nthreads, const ref_gate* gates, size_t ng, int reps, double* ms)
+{
+    if (n < 1 || n > 28 || nthreads < 1 || reps < 1) return -1;
+    for (size_t i = 0; i < ng; i++)
+        if (!valid(n, gates[i])) return -1;
+    Simulation sim((IdxType)n, (IdxType)nthreads);
+    for (size_t i = 0; i < ng; i++) {
+        const ref_gate& g = gates[i];
+        Gate gate((OP)g.op, (IdxType)g.qb[0], (IdxType)g.qb[1], (IdxType)g.qb[2],
+                  (IdxType)g.qb[3], (IdxType)g.qb[4], g.theta, g.phi, g.lambda);
+        sim.append(&gate);
+    }
+    sim.upload();
+    for (int r = 0; r < reps; r++) {
+        sim.reset_sv();
+        StdoutMute mute;
+        const auto t0 = std::chrono::steady_clock::now();
+        sim.sim();
+        const auto t1 = std::chrono::steady_clock::now();
+        ms[r] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
     return 0;
 }
 
