@@ -238,6 +238,19 @@ def bench_batch(count=512, reps=3):
             "work_share_n_ge_19": float(w[big].sum() / w.sum())}
 
 
+def bench_fp64():
+    """FP64 vector-FMA and matrix-core (MFMA 16x16x4) rates at 1, 2 and 4
+    waves per SIMD, eight independent accumulator chains each: the evidence
+    for applying fused dense blocks with v_fma_f64 rather than MFMA."""
+    import svmi
+    out = {}
+    for w in (1, 2, 4):
+        v, m = svmi.fp64_rates(w)
+        out[f"waves_per_simd_{w}"] = {"vfma_tflops": v, "mfma_tflops": m}
+        log(f"[bench] fp64 rates, {w} waves/SIMD: v_fma_f64 {v:.1f} TFLOP/s, v_mfma_f64_16x16x4 {m:.1f} TFLOP/s")
+    return out
+
+
 def bench_c1(reps=10):
     """The 12-qubit random-layer circuit (720 gates) from |0..0> plus 1024
     shots: GPU wall time per run (median of reps, each from a reset state,
@@ -610,7 +623,8 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the QFT, adjoint and batch workloads")
-    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm", "specialized", "dm", "c1"),
+    ap.add_argument("--only", choices=("qft", "adjoint", "batch", "qir", "qasm", "specialized", "dm", "c1",
+                                          "fp64"),
                     help="run only this extra workload and print its result (profiling)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="first run one device copy of this many GiB (HBM counter calibration)")
@@ -627,7 +641,7 @@ def main():
         fn = {"qft": lambda: bench_qft(33, steps=a.steps), "adjoint": bench_adjoint, "batch": bench_batch,
               "qir": lambda: bench_qir(30, reps=a.steps), "qasm": lambda: bench_qasm(reps=a.steps),
               "specialized": lambda: bench_specialized(steps=a.steps),
-              "dm": lambda: bench_dm(15, reps=a.steps), "c1": bench_c1}[a.only]
+              "dm": lambda: bench_dm(15, reps=a.steps), "c1": bench_c1, "fp64": bench_fp64}[a.only]
         log(f"[bench] only {a.only}")
         print(json.dumps({a.only: fn()}), flush=True)
         return

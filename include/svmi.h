@@ -105,11 +105,13 @@ typedef enum svmi_option {
                                  2 = the same with one 512-VGPR workgroup per CU, 0 = off (default) */
     SVMI_OPT_OP_COST = 6,     /* fusion cost model: per-op cost in FMA per amplitude (default 2;
                                  0 for SVMI_OPT_SPECIALIZE unless set)                      */
-    SVMI_OPT_COEF_BUDGET = 7, /* split a forward sweep whose op coefficients exceed this many
-                                 bytes into consecutive sweeps (default 20000, 0 = never)   */
-    SVMI_OPT_SPECIALIZE = 8   /* 1 = compile each uploaded circuit's sweeps into straight-line
+    SVMI_OPT_COEF_BUDGET = 7, /* split a forward sweep whose op records and coefficients exceed
+                                 this many bytes into consecutive sweeps (default 16000, 0 = never) */
+    SVMI_OPT_SPECIALIZE = 8,  /* 1 = compile each uploaded circuit's sweeps into straight-line
                                  kernels with hipRTC at upload (run-time compilation, seconds
                                  per circuit); 0 = interpret the sweep programs (default)     */
+    SVMI_OPT_MERGE_QUBITS = 9 /* widest dense block (1..4 register qubits) that fusion may
+                                 form from a run of gates inside one phase (default 2)       */
 } svmi_option;
 
 /* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,

@@ -69,6 +69,17 @@ svmi_status svmi_debug_plan_qir(int n, const svmi_qir_gate* gates, size_t n_gate
 svmi_status svmi_debug_jit(int n, const svmi_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
                            int compile, double* compile_ms, uint64_t* code_bytes, char* buf, size_t buflen);
 
+/* Per-step device times (ms, HIP events) of the last program the handle ran
+ * with SVMI_OPT_PROFILE set, in program order (sweeps, standalone ops,
+ * measurements); *n_steps receives their number, ms[0..capacity) the times. */
+svmi_status svmi_debug_step_times(svmi_handle h, double* ms, size_t capacity, uint64_t* n_steps);
+
+/* FP64 issue rates of the device (calibration): TFLOP/s of plain vector
+ * FMAs (v_fma_f64) and of FP64 matrix-core products (v_mfma_f64_16x16x4f64),
+ * each with eight independent accumulator chains, 256-thread workgroups and
+ * waves_per_simd (1..8) workgroups per CU. */
+svmi_status svmi_debug_fp64_rates(int device, int waves_per_simd, double* vfma_tflops, double* mfma_tflops);
+
 /* Planner summary: number of full-state passes and a text description
  * (truncated to buflen-1 bytes; buf may be NULL). */
 svmi_status svmi_debug_plan(int n, const svmi_gate* gates, size_t n_gates,

@@ -304,7 +304,8 @@ svmi_status verify_batch(const BatchPlan& bp)
             const bool sweep = b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ;
             if (b.kind != L.kind || (sweep && bp.sweeps[b.sweep].k != L.k) || b.init_zero != L.init ||
                 (b.kind == BSTEP_ADJ && bp.sweeps[b.sweep].n_grad > L.max_grad) ||
-                (sweep && bp.sweeps[b.sweep].n_xf > L.max_xf))
+                (sweep && bp.sweeps[b.sweep].n_xf > L.max_xf) ||
+                (sweep && bp.sweeps[b.sweep].max_dense > L.max_dense))
                 return fail(SVMI_ERR_STATE, "batch plan: mixed launch");
             nt += b.n_tasks;
         }
@@ -332,10 +333,13 @@ svmi_status verify_batch(const BatchPlan& bp)
                 return fail(SVMI_ERR_STATE, "batch plan: sweep does not match the circuit size");
             if (b.kind == BSTEP_ADJ && b.partial + sw.n_tiles * sw.n_grad > bp.n_grad_partials)
                 return fail(SVMI_ERR_STATE, "batch plan: bad gradient partial range");
-            if (sw.op_begin > sw.op_end || sw.op_end > bp.ops.size() || sw.n_xf > uint32_t(kMaxPhaseFactors))
+            if (sw.op_begin > sw.op_end || sw.op_end > bp.ops.size() || sw.n_xf > uint32_t(kMaxXfSlots) ||
+                16u * sw.n_lt > sw.n_xf || sw.lt_coef + 16ull * sw.n_lt > bp.coef.size())
                 return fail(SVMI_ERR_STATE, "batch plan: bad sweep op range");
             for (uint32_t o = sw.op_begin; o < sw.op_end; o++)
-                if (bp.ops[o].h.type == DOP_PHASESET && bp.ops[o].h.s1 && bp.ops[o].h.grad >= sw.n_xf)
+                if ((has_tile_factor(bp.ops[o].h) && bp.ops[o].h.grad >= sw.n_xf) ||
+                    (bp.ops[o].h.type == DOP_PHASETAB && bp.ops[o].h.gmask != kLaneTabGlobal &&
+                     uint64_t(bp.ops[o].h.gmask) + 16ull * bp.ops[o].h.s0 > sw.n_xf))
                     return fail(SVMI_ERR_STATE, "batch plan: bad phase factor slot");
             for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
                 const PhaseInfo& ph = bp.phases[p];
@@ -447,6 +451,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
             sw.op_begin += op_base;
             sw.op_end += op_base;
             sw.hitab += ht_base;
+            sw.lt_coef += coef_base;
             bp.sweeps.push_back(sw);
         }
         bp.coef.insert(bp.coef.end(), pr.coef.begin(), pr.coef.end());
@@ -558,6 +563,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
             const uint32_t k = tile_k(b);
             const uint32_t ng = b.kind == BSTEP_ADJ ? bp.sweeps[b.sweep].n_grad : 0u;
             const uint32_t nx = (b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ) ? bp.sweeps[b.sweep].n_xf : 0u;
+            const uint32_t md = (b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ) ? bp.sweeps[b.sweep].max_dense : 0u;
             if (!bp.launches.empty() && bp.launches.back().step_end == idx &&
                 bp.launches.back().step_begin >= bp.level_begin.back() && bp.launches.back().kind == b.kind &&
                 bp.launches.back().k == k && bp.launches.back().init == b.init_zero) {
@@ -566,6 +572,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
                 L.n_tasks += b.n_tasks;
                 L.max_grad = std::max(L.max_grad, ng);
                 L.max_xf = std::max(L.max_xf, nx);
+                L.max_dense = std::max(L.max_dense, md);
             } else {
                 BatchLaunch L;
                 std::memset(&L, 0, sizeof(L));
@@ -575,6 +582,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
                 L.k = k;
                 L.max_grad = ng;
                 L.max_xf = nx;
+                L.max_dense = md;
                 L.init = b.init_zero;
                 L.task_begin = b.task_begin;
                 L.n_tasks = b.n_tasks;
@@ -623,6 +631,7 @@ struct svmi_sim {
     bool profile = false;
     int pipeline = 0;
     std::vector<hipEvent_t> prof_events;
+    std::vector<double> step_ms;   // with profiling: per-step times of the last program run
     svmi_stats stats{};
     // backward program of the uploaded circuit for the last parameter map
     // (reused while the circuit, the map and the options stay the same)
@@ -729,9 +738,11 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
     h->stats.sweeps += prog.steps.size();
     if (prof) {
         SVMI_HIP(hipStreamSynchronize(h->stream));
+        h->step_ms.assign(prog.steps.size(), 0.0);
         for (size_t i = 0; i < prog.steps.size(); i++) {
             float ms = 0.f;
             SVMI_HIP(hipEventElapsedTime(&ms, h->prof_events[2 * i], h->prof_events[2 * i + 1]));
+            h->step_ms[i] = ms;
             if (prog.steps[i].kind == Step::SWEEP) {
                 h->stats.sweep_launches++;
                 h->stats.sweep_ms += ms;
@@ -1373,6 +1384,10 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
         h->pipeline = int(value);
         return SVMI_OK;
     case SVMI_OPT_SPECIALIZE: h->specialize = value != 0; break;
+    case SVMI_OPT_MERGE_QUBITS:
+        if (value < 1 || value > kRegBits) return fail(SVMI_ERR_ARG, "merge qubits must be in [1, 4]");
+        h->opt.merge_qubits = int(value);
+        break;
     default: return fail(SVMI_ERR_ARG, "unknown option " + std::to_string(option));
     }
     h->adj_valid = false;
@@ -1460,8 +1475,10 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
     for (uint32_t i = 0; i < sw.n_grad; i++) gsum_tile[i] = 0.0;
     const uint64_t base = tile_base(sw, tile);
     std::vector<c64> xf(std::max<uint32_t>(sw.n_xf, 1));
+    const uint64_t xbits = ext_index(sw, base);
+    for (uint32_t i = 0; i < 16u * sw.n_lt; i++) xf[sw.n_xf - 16u * sw.n_lt + i] = coef[sw.lt_coef + i];
     for (uint32_t o = sw.op_begin; o < sw.op_end; o++)
-        if (ops[o].h.type == DOP_PHASESET && ops[o].h.s1) xf[ops[o].h.grad] = phase_factor(ops[o].h, coef, base);
+        if (has_tile_factor(ops[o].h)) xf[ops[o].h.grad] = tile_factor(ops[o].h, coef, base, xbits);
     for (uint32_t l = 0; l < tsize; l++) {
         gidx[l] = base | hitab[sw.hitab + (l >> sw.c)] | (l & cmask);
         t[l] = init_zero ? c64{gidx[l] == 0 ? 1.0 : 0.0, 0.0} : s0[gidx[l]];
@@ -1480,7 +1497,7 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
                 if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG)
                     gsum_tile[op.h.grad] += exec_grad_op<kRegBits, kRegsAdj>(op.h, &op, a, coef, goff, off, base);
                 else
-                    exec_state_op(op.h, &op, a, coef, goff, off, base, xf.data());
+                    exec_state_op(op.h, &op, a, coef, goff, g, off, base, xf.data());
             }
             for (int r = 0; r < kRegs; r++) t[goff | off[r]] = a[r];
         }
@@ -1819,6 +1836,14 @@ svmi_status svmi_debug_plan_qir(int n, const svmi_qir_gate* gates, size_t n_gate
         std::memcpy(buf, d.data(), m);
         buf[m] = 0;
     }
+    return SVMI_OK;
+}
+
+svmi_status svmi_debug_step_times(svmi_handle h, double* ms, size_t capacity, uint64_t* n_steps)
+{
+    if (!h || !n_steps || (capacity && !ms)) return fail(SVMI_ERR_ARG, "null argument");
+    *n_steps = h->step_ms.size();
+    for (size_t i = 0; i < capacity && i < h->step_ms.size(); i++) ms[i] = h->step_ms[i];
     return SVMI_OK;
 }
 

@@ -117,10 +117,10 @@ SVMI_HD inline void dispatch_srn(int s, c64 (&a)[N])
 }
 
 template <bool kThree, int N>
-SVMI_HD inline void dispatch_lift(int s, c64 (&a)[N], const c64* __restrict__ uv)
+SVMI_HD inline void dispatch_lift(const DOpHdr& h, c64 (&a)[N])
 {
-    const double u = uv->re, l = uv->im;
-    switch (s) {
+    const double u = lift_u(h), l = lift_l(h);
+    switch (h.s0) {
         SVMI_CASE_ID(0, 300 + 10 * kThree, (apply_lift<0, kThree, N>(a, u, l)))
         SVMI_CASE_ID(1, 301 + 10 * kThree, (apply_lift<1, kThree, N>(a, u, l)))
         SVMI_CASE_ID(2, 302 + 10 * kThree, (apply_lift<2, kThree, N>(a, u, l)))
@@ -230,6 +230,40 @@ SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[N], const c64* __re
         if ((rsel >> (r & (kRegs - 1))) & 1u) a[r] = cmul(a[r], ph);
 }
 
+// DOP_PHASETAB: the lane's product of nibble-table entries (times the tile
+// factor), applied to the registers of rsel.
+template <int N>
+SVMI_HD inline void apply_phasetab(const DOpHdr& h, c64 (&a)[N], const c64* __restrict__ coef, uint32_t grp,
+                                   const c64* xf)
+{
+    c64 ph = c64{1.0, 0.0};
+    if (h.s1) ph = xf[h.grad];
+    const c64* T = h.gmask == kLaneTabGlobal ? coef + h.coef : xf + h.gmask;
+    for (uint32_t j = 0; j < h.s0; j++) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
+    const uint32_t rsel = h.rsel;
+#pragma unroll
+    for (int r = 0; r < N; r++)
+        if ((rsel >> (r & (kRegs - 1))) & 1u) a[r] = cmul(a[r], ph);
+}
+
+// The tile base's bits outside the tile, packed in ascending qubit order.
+SVMI_HD inline uint64_t ext_index(const SweepInfo& sw, uint64_t base)
+{
+    uint64_t x = 0;
+    int j = 0;
+    for (uint64_t m = sw.extmask; m; m &= m - 1, j++) x |= ((base >> __builtin_ctzll(m)) & 1ull) << j;
+    return x;
+}
+
+// Factor slot of a DOP_PHASETAB op for the tile with external bits x.
+SVMI_HD inline c64 phasetab_factor(const DOpHdr& h, const c64* __restrict__ coef, uint64_t x)
+{
+    const c64* E = coef + h.coef + 16u * h.s0;
+    c64 f{1.0, 0.0};
+    for (uint32_t j = 0; j < h.smask; j++) f = cmul(f, E[16u * j + uint32_t((x >> (4u * j)) & 15u)]);
+    return f;
+}
+
 // Factor slot of a DOP_PHASESET op for the tile at `base`: the product of
 // its h.s1 entries that follow the per-lane ones.
 SVMI_HD inline c64 phase_factor(const DOpHdr& h, const c64* __restrict__ coef, uint64_t base)
@@ -247,11 +281,19 @@ SVMI_HD inline c64 phase_factor(const DOpHdr& h, const c64* __restrict__ coef, u
     return c64{pr, pi};
 }
 
-// h = the op's header (prefetched), op = the op in memory (diagonal tail).
+// The tile factor of op h (DOP_PHASESET / DOP_PHASETAB with s1) for the
+// tile at `base` with external bits x.
+SVMI_HD inline c64 tile_factor(const DOpHdr& h, const c64* __restrict__ coef, uint64_t base, uint64_t x)
+{
+    return h.type == DOP_PHASETAB ? phasetab_factor(h, coef, x) : phase_factor(h, coef, base);
+}
+
+// h = the op's header (prefetched), op = the op in memory (diagonal tail),
+// grp = the group index (goff with the register bits squeezed out).
 template <int kMaxQ = kRegBits, int N = kRegs>
 SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[N],
-                                  const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
-                                  uint64_t base, const c64* xf)
+                                  const c64* __restrict__ coef, uint32_t goff, uint32_t grp,
+                                  const uint32_t (&off)[kRegs], uint64_t base, const c64* xf)
 {
     switch (h.type) {
         SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, (dispatch_dense<kMaxQ, N>(h.smask, a, coef + h.coef)))
@@ -263,9 +305,10 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag<N>(*op, a, coef, goff, off, base))
         SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset<N>(h, a, coef, goff, base, xf))
         SVMI_CASE_ID(DOP_CDENSE, DOP_CDENSE + 100, dispatch_cdense<N>(h, a, coef, goff, base))
-        SVMI_CASE_ID(DOP_LIFT2, DOP_LIFT2 + 100, (dispatch_lift<false, N>(h.s0, a, coef + h.coef)))
-        SVMI_CASE_ID(DOP_LIFT3, DOP_LIFT3 + 100, (dispatch_lift<true, N>(h.s0, a, coef + h.coef)))
+        SVMI_CASE_ID(DOP_LIFT2, DOP_LIFT2 + 100, (dispatch_lift<false, N>(h, a)))
+        SVMI_CASE_ID(DOP_LIFT3, DOP_LIFT3 + 100, (dispatch_lift<true, N>(h, a)))
         SVMI_CASE_ID(DOP_RTAB, DOP_RTAB + 100, apply_rtab<N>(a, coef + h.coef + 16u * rtab_variant(h.emask, base)))
+        SVMI_CASE_ID(DOP_PHASETAB, DOP_PHASETAB + 100, apply_phasetab<N>(h, a, coef, grp, xf))
     default: break;
     }
 }
@@ -318,11 +361,14 @@ SVMI_HD inline uint32_t group_offset(const PhaseInfo& ph, uint32_t g)
 // next 4, so strided register gathers spread over the LDS banks).
 SVMI_HD inline uint32_t lds_slot(uint32_t l) { return l ^ ((l >> 4) & 15u); }
 
-// Tile base: pdep of the tile id over the sweep's external positions.
+// Tile base: the tile id's bits deposited on the sweep's external positions,
+// low bits on extmask & ~exthi, high bits on exthi (each ascending).
 SVMI_HD inline uint64_t tile_base(const SweepInfo& sw, uint64_t tile)
 {
     uint64_t base = 0;
-    for (uint64_t m = sw.extmask; m; m &= m - 1, tile >>= 1)
+    for (uint64_t m = sw.extmask & ~sw.exthi; m; m &= m - 1, tile >>= 1)
+        base |= (tile & 1ull) << __builtin_ctzll(m);
+    for (uint64_t m = sw.exthi; m; m &= m - 1, tile >>= 1)
         base |= (tile & 1ull) << __builtin_ctzll(m);
     return base;
 }

@@ -107,7 +107,19 @@ enum DOpType : uint16_t {
     DOP_RTAB = 14,       // diagonal over the register group: a[r] *= T[r], T = coef + 16*v,
                          // v = the tile base's bits at the positions of emask (ascending),
                          // so a table may depend on up to kMaxTabExt qubits outside the tile
+    DOP_PHASETAB = 15,   // product of controlled phases from nibble tables, applied to the
+                         // registers rsel: ph = T_0[g & 15] * ... * T_{s0-1}[(g >> 4(s0-1)) & 15]
+                         // for the group index g (bit m = the m-th tile bit outside the
+                         // registers), times the tile factor xf[grad] if s1; that factor is
+                         // E_0[x & 15] * ... * E_{smask-1}[...] for x = the tile base's bits
+                         // outside the tile (ext_index); T_j, E_j are 16 c64 each at coef;
+                         // gmask = the factor-area slot where the tile keeps the T_j, or
+                         // kLaneTabGlobal to read them from coef
 };
+constexpr uint32_t kLaneTabGlobal = 0xffffffffu;
+constexpr int kMaxXfSlots = 1024;      // factor area per sweep: 16 KiB of LDS
+constexpr int kPhaseTabMaxLane = 3;    // nibble tables over the group index (9 group bits)
+constexpr int kPhaseTabMaxExt = 16;    // nibble tables over the bits outside the tile
 constexpr int kMaxTabExt = 4;          // qubits outside the tile one register table may depend on
 
 // One controlled phase of a DOP_PHASESET (2 c64 slots in the coefficient
@@ -142,6 +154,38 @@ struct DOpHdr {
     uint32_t gmask;      // CPHASE/DIAG/CDENSE: non-register tile-local bits that must be 1
     uint64_t emask;      // CPHASE/DIAG/CDENSE: physical bits outside the tile that must be 1
 };
+// DOP_LIFT2 / DOP_LIFT3 also carry their two shear factors in the header
+// (u in emask's bits, l in gmask | rsel << 32), so the interpreted sweeps
+// apply a lift without waiting for a coefficient load; coef still points at
+// {u, l} in the coefficient array for the specialised kernels.
+SVMI_HD inline uint64_t lift_bits(double x)
+{
+    uint64_t b;
+    __builtin_memcpy(&b, &x, sizeof(b));
+    return b;
+}
+SVMI_HD inline double lift_val(uint64_t b)
+{
+    double x;
+    __builtin_memcpy(&x, &b, sizeof(x));
+    return x;
+}
+SVMI_HD inline void lift_set(DOpHdr& h, double u, double l)
+{
+    h.emask = lift_bits(u);
+    const uint64_t lb = lift_bits(l);
+    h.gmask = uint32_t(lb);
+    h.rsel = uint32_t(lb >> 32);
+}
+SVMI_HD inline double lift_u(const DOpHdr& h) { return lift_val(h.emask); }
+SVMI_HD inline double lift_l(const DOpHdr& h) { return lift_val(uint64_t(h.gmask) | uint64_t(h.rsel) << 32); }
+
+// Ops whose product of phases has a per-tile factor (LDS slot h.grad).
+SVMI_HD inline bool has_tile_factor(const DOpHdr& h)
+{
+    return (h.type == DOP_PHASESET || h.type == DOP_PHASETAB) && h.s1;
+}
+
 struct DOp {
     DOpHdr h;
     uint32_t nq;         // DIAG: number of qubits
@@ -165,11 +209,19 @@ struct SweepInfo {
     uint32_t n_grad;     // adjoint: accumulators used by this sweep
     uint32_t grad_base;  // adjoint: first global accumulator of this sweep
     uint64_t extmask;    // physical qubits outside the tile
+    uint64_t exthi;      // those of them that are the tile id's most significant bits (tile id
+                         // bit j <-> the j-th qubit of extmask & ~exthi, then of exthi, ascending):
+                         // the qubits that select register-table variants, so consecutive tiles
+                         // read the same variants
     uint64_t n_tiles;
-    uint32_t n_xf;       // tile-uniform phase factors (LDS slots) of this sweep
+    uint32_t n_xf;       // LDS c64 slots of the sweep's factor area: tile-uniform phase
+                         // factors, then (last 16 * n_lt) the lane tables of its DOP_PHASETAB ops
     uint32_t op_begin;   // the sweep's ops are [op_begin, op_end)
     uint32_t op_end;
-    uint32_t pad;
+    uint32_t max_dense;  // register qubits of the sweep's widest DOP_DENSE (0 = none): sweeps
+                         // with max_dense <= 2 run a kernel compiled without wider blocks
+    uint32_t lt_coef;    // coef offset of a contiguous copy of those lane tables (16 c64 each),
+    uint32_t n_lt;       // which every tile copies into the end of its factor area
 };
 
 // A standalone op (run by its own kernel on the whole state, outside sweeps).

@@ -114,6 +114,15 @@ void emit_state_op(std::ostringstream& o, const DOp& d, const PhaseInfo& ph)
         o << "        }\n";
         return;
     }
+    case DOP_PHASETAB: {
+        o << "        {\n            c64 ph = " << (h.s1 ? "xf[" + std::to_string(h.grad) + "]" : "c64{1.0, 0.0}")
+          << ";\n";
+        for (uint32_t j = 0; j < h.s0; j++)
+            o << "            ph = cmul(ph, coef[" << (h.coef + 16u * j) << "u + ((tid >> " << 4 * j << ") & 15u)]);\n";
+        for (int r : selected(h.rsel)) o << "            a[" << r << "] = cmul(a[" << r << "], ph);\n";
+        o << "        }\n";
+        return;
+    }
     default: o << "        #error unsupported op type " << h.type << "\n"; return;
     }
 }
@@ -133,20 +142,27 @@ void emit_sweep(std::ostringstream& o, const Program& prog, uint32_t i)
       << "    uint64_t base = 0;\n";
     // tile base: the tile id's bits deposited on the qubits outside the tile
     int j = 0;
-    for (int p = 0; p < 64; p++)
-        if ((sw.extmask >> p) & 1) o << "    base |= ((tile >> " << j++ << ") & 1ull) << " << p << ";\n";
+    for (const uint64_t m : {sw.extmask & ~sw.exthi, sw.exthi})
+        for (int p = 0; p < 64; p++)
+            if ((m >> p) & 1) o << "    base |= ((tile >> " << j++ << ") & 1ull) << " << p << ";\n";
     const uint32_t cmask = (1u << sw.c) - 1u;
     o << "#pragma unroll\n    for (int j = 0; j < " << kRegs << "; j++) {\n"
       << "        const uint32_t l = tid + " << nt << "u * j;\n"
       << "        t0[lds_slot(l)] = s[base | ht[" << sw.hitab << "u + (l >> " << sw.c << ")] | (l & " << cmask
       << "u)];\n    }\n";
     // tile-uniform phase factors
+    if (sw.n_xf) {
+        o << "    uint64_t xbits = 0;\n";
+        int jx = 0;
+        for (int p = 0; p < 64; p++)
+            if ((sw.extmask >> p) & 1) o << "    xbits |= ((base >> " << p << ") & 1ull) << " << jx++ << ";\n";
+    }
     for (uint32_t op = sw.op_begin; op < sw.op_end; op++) {
         const DOpHdr& h = prog.ops[op].h;
-        if (h.type == DOP_PHASESET && h.s1) {
+        if (has_tile_factor(h)) {
             o << "    if (tid == " << (h.grad % nt) << "u) {\n        constexpr DOpHdr H = ";
             emit_hdr(o, h);
-            o << ";\n        xf[" << h.grad << "] = phase_factor(H, coef, base);\n    }\n";
+            o << ";\n        xf[" << h.grad << "] = tile_factor(H, coef, base, xbits);\n    }\n";
         }
     }
     o << "    __syncthreads();\n";

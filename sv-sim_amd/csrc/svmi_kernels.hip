@@ -94,7 +94,7 @@ __device__ __attribute__((always_inline)) inline double wave_sum_lane63(double v
 // kernel: blockDim >= nt) the others only join the barriers.  Without kIdle
 // every thread is active, which keeps the phase body free of divergence
 // (and its register groups out of scratch).  Ends with a barrier.
-template <int NS, bool kIdle>
+template <int NS, bool kIdle, int kMaxQ>
 __device__ __attribute__((always_inline)) inline void
 tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
             const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops, const c64* __restrict__ coef,
@@ -137,7 +137,7 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
                         continue;
                     }
                 }
-                exec_state_op<NS == 2 ? kMaxDenseQubitsAdj : kRegBits>(h, ops + o, a, coef, goff, off, base, xf);
+                exec_state_op<kMaxQ>(h, ops + o, a, coef, goff, tid, off, base, xf);
             }
 #pragma unroll
             for (int r = 0; r < kRegs; r++) t0[lds_slot(goff | off[r])] = a[r];
@@ -153,9 +153,13 @@ tile_factors(c64* xf, const SweepInfo& sw, const DOp* __restrict__ ops, const c6
              uint64_t base, uint32_t tid)
 {
     if (sw.n_xf == 0) return;
+    // the sweep's lane tables, then the tile's factors
+    const uint32_t nlt = 16u * sw.n_lt, lt0 = sw.n_xf - nlt;
+    for (uint32_t i = tid; i < nlt; i += blockDim.x) xf[lt0 + i] = coef[sw.lt_coef + i];
+    const uint64_t x = ext_index(sw, base);
     for (uint32_t o = sw.op_begin + tid; o < sw.op_end; o += blockDim.x) {
         const DOpHdr h = ops[o].h;
-        if (h.type == DOP_PHASESET && h.s1) xf[h.grad] = phase_factor(h, coef, base);
+        if (has_tile_factor(h)) xf[h.grad] = tile_factor(h, coef, base, x);
     }
 }
 
@@ -163,7 +167,7 @@ tile_factors(c64* xf, const SweepInfo& sw, const DOp* __restrict__ ops, const c6
 // gradient accumulators.
 // One tile of one sweep: load, phases, store (and for NS = 2 the tile's
 // gradient partials).  init_zero: the tile is generated as |0..0>.
-template <int NS, bool kIdle>
+template <int NS, bool kIdle, int kMaxQ = (NS == 2 ? kMaxDenseQubitsAdj : kRegBits)>
 __device__ __attribute__((always_inline)) inline void
 sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
            const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
@@ -194,7 +198,7 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
         for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
     tile_factors(xf, sw, ops, coef, base, tid);
     __syncthreads();
-    tile_phases<NS, kIdle>(t0, gacc, xf, sw, phases, ops, coef, base, tid, nt);
+    tile_phases<NS, kIdle, kMaxQ>(t0, gacc, xf, sw, phases, ops, coef, base, tid, nt);
     if (active) {
         tile_store<J>(t0, 0u, s0, sw, ht, tile, tid, nt);
         if constexpr (NS == 2) tile_store<J>(t0, tsize, s1, sw, ht, tile, tid, nt);
@@ -215,6 +219,9 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
 // stagger_lo..stagger_hi: workgroups of the first resident wave that start
 // later (by `stagger` sleeps of 127*64 cycles), so that the co-resident
 // workgroups of a CU alternate loading and computing.
+// kMaxQ: widest dense block compiled in (sweeps with blocks of at most two
+// register qubits run the kMaxQ = 2 build, which holds fewer registers).
+template <int kMaxQ>
 __global__ void __launch_bounds__(kMaxTileThreads, 1)
 k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
@@ -223,7 +230,7 @@ k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__
     extern __shared__ __align__(16) unsigned char smem[];
     if (blockIdx.x >= stagger_lo && blockIdx.x < stagger_hi)
         for (uint32_t i = 0; i < stagger; i++) __builtin_amdgcn_s_sleep(127);
-    sweep_tile<1, false>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x,
+    sweep_tile<1, false, kMaxQ>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x,
                          smem, false);
 }
 
@@ -256,7 +263,7 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
         const uint64_t tn = t + gridDim.x;
         const bool more = tn < n_tiles;
         if (more) tile_fetch(nx, s0, sw, ht, tile_offset + tn, tid, nt, false);
-        tile_phases<1, false>(t0, nullptr, xf, sw, phases, ops, coef, base, tid, nt);
+        tile_phases<1, false, kRegBits>(t0, nullptr, xf, sw, phases, ops, coef, base, tid, nt);
         tile_store<kRegs>(t0, 0u, s0, sw, ht, tile_offset + t, tid, nt);
         if (!more) break;
         __syncthreads();  // the store's LDS reads finish before the next put
@@ -309,7 +316,7 @@ __device__ inline uint32_t batch_step_of(const BatchStep* __restrict__ steps, ui
         const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab
 
 // kInit: the steps are first forward sweeps (tiles generated as |0..0>).
-template <bool kInit>
+template <bool kInit, int kMaxQ>
 __global__ void __launch_bounds__(kMaxTileThreads, 1)
 k_batch_fwd(SVMI_BATCH_TABLES, uint32_t s0, uint32_t s1, uint64_t task0)
 {
@@ -317,7 +324,7 @@ k_batch_fwd(SVMI_BATCH_TABLES, uint32_t s0, uint32_t s1, uint64_t task0)
     const uint64_t t = task0 + blockIdx.x;
     const BatchStep st = steps[batch_step_of(steps, s0, s1, t)];
     const SweepInfo sw = sweeps[st.sweep];
-    sweep_tile<1, false>(circuits[st.circuit].phi, nullptr, sw, phases, ops, coef, hitab, nullptr,
+    sweep_tile<1, false, kMaxQ>(circuits[st.circuit].phi, nullptr, sw, phases, ops, coef, hitab, nullptr,
                          t - st.task_begin, blockDim.x, smem, kInit);
 }
 
@@ -396,14 +403,13 @@ hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L
         const uint32_t cnt = uint32_t(L.n_tasks - t0 < max_grid ? L.n_tasks - t0 : max_grid);
         const uint64_t first = L.task_begin + t0;
         switch (L.kind) {
-        case BSTEP_FWD:
-            if (L.init)
-                hipLaunchKernelGGL(k_batch_fwd<true>, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits,
-                                   a.sweeps, a.phases, a.ops, a.coef, a.hitab, L.step_begin, L.step_end, first);
-            else
-                hipLaunchKernelGGL(k_batch_fwd<false>, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits,
-                                   a.sweeps, a.phases, a.ops, a.coef, a.hitab, L.step_begin, L.step_end, first);
+        case BSTEP_FWD: {
+            auto* fn = L.init ? (L.max_dense <= 2 ? k_batch_fwd<true, 2> : k_batch_fwd<true, kRegBits>)
+                              : (L.max_dense <= 2 ? k_batch_fwd<false, 2> : k_batch_fwd<false, kRegBits>);
+            hipLaunchKernelGGL(fn, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits, a.sweeps, a.phases,
+                               a.ops, a.coef, a.hitab, L.step_begin, L.step_end, first);
             break;
+        }
         case BSTEP_ADJ:
             hipLaunchKernelGGL(k_batch_adj, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits, a.sweeps,
                                a.phases, a.ops, a.coef, a.hitab, a.grad_partials, L.step_begin, L.step_end, first);
@@ -476,8 +482,8 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
                 hi = uint32_t(res);
                 sl = uint32_t(pipeline - 2);
             }
-            hipLaunchKernelGGL(k_sweep1, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, sw, phases, ops, coef,
-                               hitab, t0, lo, hi, sl);
+            hipLaunchKernelGGL(sw.max_dense <= 2 ? k_sweep1<2> : k_sweep1<kRegBits>, dim3(uint32_t(cnt)), dim3(nt),
+                               smem, st, s0, sw, phases, ops, coef, hitab, t0, lo, hi, sl);
         }
         else
             hipLaunchKernelGGL(k_sweep2, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases, ops,
@@ -501,8 +507,11 @@ hipError_t init_sweep_kernels()
     size_t want = size_t(optin);
     if (want > 160 * 1024) want = 160 * 1024;
     if (want > 64 * 1024) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1),
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<kRegBits>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<2>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e == hipSuccess)
             e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1p<false>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
@@ -518,9 +527,13 @@ hipError_t init_sweep_kernels()
         }
     }
     if (want > 64 * 1024 &&
-        (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<true>),
+        (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<true, kRegBits>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<false>),
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<false, kRegBits>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<true, 2>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
+         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<false, 2>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
          hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_adj), hipFuncAttributeMaxDynamicSharedMemorySize,
                              int(want)) != hipSuccess)) {

@@ -158,6 +158,13 @@ struct Encoder {
             groups[g].second.push_back(e);
         }
         for (auto& gr : groups) {
+            // entries whose condition is one group bit, or one qubit outside
+            // the tile, become nibble tables (DOP_PHASETAB); the rest stay a
+            // list evaluated per lane
+            std::vector<PhaseEntry> rest;
+            encode_phasetab(gr.first, gr.second, tloc, slot_of, rest);
+            if (rest.empty()) continue;
+            gr.second.swap(rest);
             // entries with no condition inside the tile depend on the tile
             // only: their product is formed once per tile (factor slot)
             std::vector<PhaseEntry> lane, uni;
@@ -184,6 +191,73 @@ struct Encoder {
                 }
             prog.ops.push_back(d);
         }
+    }
+
+    // DOP_PHASETAB for the entries of one register condition (rsel) that
+    // nibble tables can hold; the others are returned in `rest`.  Group bit
+    // m (the m-th tile bit outside the registers, ascending) is bit m % 4 of
+    // lane table m / 4; the m-th qubit outside the tile (ascending) is bit
+    // m % 4 of external table m / 4; an entry without any condition outside
+    // the registers multiplies every entry of the first table.
+    void encode_phasetab(uint32_t rsel, const std::vector<PhaseEntry>& in, const int* tloc, const int* slot_of,
+                         std::vector<PhaseEntry>& rest)
+    {
+        int k = 0;
+        for (int p = 0; p < prog.n; p++)
+            if (tloc[p] >= 0) k++;
+        int gidx[64], xidx[64];
+        int ng = 0, nx = 0;
+        for (int l = 0; l < 64; l++) gidx[l] = (l < k && slot_of[l] < 0) ? ng++ : -1;
+        for (int p = 0; p < 64; p++) xidx[p] = (p < prog.n && tloc[p] < 0) ? nx++ : -1;
+        std::vector<PhaseEntry> lane, uni, cst;
+        for (const PhaseEntry& e : in) {
+            const int pg = __builtin_popcount(e.gmask), px = __builtin_popcountll(e.emask);
+            if (pg == 1 && px == 0) lane.push_back(e);
+            else if (pg == 0 && px == 1) uni.push_back(e);
+            else if (pg == 0 && px == 0) cst.push_back(e);
+            else rest.push_back(e);
+        }
+        const bool has_x = !uni.empty() || (!cst.empty() && lane.empty());
+        if (lane.size() + uni.size() + cst.size() < 2 || (has_x && (nx == 0 || n_xf >= uint32_t(kMaxPhaseFactors))) ||
+            (!cst.empty() && lane.empty() && nx == 0)) {
+            rest = in;
+            return;
+        }
+        int nlt = 0, nxt = 0;
+        for (const PhaseEntry& e : lane) nlt = std::max(nlt, gidx[__builtin_ctz(e.gmask)] / 4 + 1);
+        for (const PhaseEntry& e : uni) nxt = std::max(nxt, xidx[__builtin_ctzll(e.emask)] / 4 + 1);
+        if (!cst.empty() && nlt == 0 && nxt == 0) nxt = 1;
+        if (nlt > kPhaseTabMaxLane || nxt > kPhaseTabMaxExt) {
+            rest = in;
+            return;
+        }
+        std::vector<c64> T(size_t(16) * (nlt + nxt), c64{1.0, 0.0});
+        auto mul_bit = [&](size_t table, int bit, double re, double im) {
+            for (int v = 0; v < 16; v++)
+                if ((v >> bit) & 1) T[table * 16 + v] = cmul(T[table * 16 + v], c64{re, im});
+        };
+        for (const PhaseEntry& e : lane) {
+            const int m = gidx[__builtin_ctz(e.gmask)];
+            mul_bit(size_t(m / 4), m % 4, e.ph_re, e.ph_im);
+        }
+        for (const PhaseEntry& e : uni) {
+            const int m = xidx[__builtin_ctzll(e.emask)];
+            mul_bit(size_t(nlt + m / 4), m % 4, e.ph_re, e.ph_im);
+        }
+        for (const PhaseEntry& e : cst)
+            for (int v = 0; v < 16; v++) T[v] = cmul(T[v], c64{e.ph_re, e.ph_im});
+        DOp d;
+        std::memset(&d, 0, sizeof(d));
+        d.targets = prog.n_states == 2 ? DOP_TO_BOTH : DOP_TO_PSI;
+        d.h.type = DOP_PHASETAB;
+        d.h.rsel = rsel;
+        d.h.s0 = uint32_t(nlt);
+        d.h.s1 = nxt ? 1u : 0u;
+        d.h.smask = uint32_t(nxt);
+        d.h.grad = nxt ? n_xf++ : 0u;
+        d.h.gmask = kLaneTabGlobal;   // the sweep may stage the lane tables in LDS
+        d.h.coef = push_coef(T);
+        prog.ops.push_back(d);
     }
 
     // Split the qubits of a condition ("all these bits are 1") into register
@@ -736,6 +810,7 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
                 e.d.targets = two_state ? DOP_TO_BOTH : DOP_TO_PSI;
                 e.d.h.type = it.three ? DOP_LIFT3 : DOP_LIFT2;
                 e.d.h.s0 = uint32_t(it.slot);
+                lift_set(e.d.h, it.u, it.l);
                 e.coef.push_back(c64{it.u, it.l});
                 out.push_back(std::move(e));
                 cost += (it.three ? 3 : 2) + overhead;
@@ -910,7 +985,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         std::vector<const SOp*> merged;
         {
             RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged, sowned,
-                         prog.n_states == 2 ? kMaxDenseQubitsAdj : kRegBits, opt.op_overhead, false, {}, 0};
+                         std::min(prog.n_states == 2 ? kMaxDenseQubitsAdj : kRegBits, opt.merge_qubits),
+                         opt.op_overhead, false, {}, 0};
             for (const SOp* s : ph) rm.add(s);
             rm.flush();
         }
@@ -919,7 +995,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             const bool two = prog.n_states == 2;
             std::vector<const SOp*> merged_rot;
             RunMerger rm{tloc, slot_of, phys_of_slot, owned, merged_rot, sowned,
-                         two ? kMaxDenseQubitsAdj : kRegBits, opt.op_overhead, true, {}, 0};
+                         std::min(two ? kMaxDenseQubitsAdj : kRegBits, opt.merge_qubits), opt.op_overhead, true,
+                         {}, 0};
             for (const SOp* s : ph) rm.add(s);
             rm.flush();
             std::vector<PEmit> em;
@@ -978,8 +1055,33 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         remaining.swap(rest);
     }
     sw.phase_end = uint32_t(prog.phases.size());
+    // qubits that select register-table variants become the tile id's high
+    // bits: workgroups running at the same time then read the same variants
+    for (size_t o = sw.op_begin; o < prog.ops.size(); o++)
+        if (prog.ops[o].h.type == DOP_RTAB) sw.exthi |= prog.ops[o].h.emask;
     sw.n_grad = n_grad_sweep;
+    // lane tables of the sweep's DOP_PHASETAB ops: one contiguous copy that
+    // every tile stages in LDS after its factor slots (if the area has room)
     sw.n_xf = enc.n_xf;
+    {
+        uint32_t ntab = 0;
+        for (size_t o = sw.op_begin; o < prog.ops.size(); o++)
+            if (prog.ops[o].h.type == DOP_PHASETAB) ntab += prog.ops[o].h.s0;
+        if (ntab && sw.n_xf + 16u * ntab <= uint32_t(kMaxXfSlots)) {
+            sw.lt_coef = uint32_t(prog.coef.size());
+            sw.n_lt = ntab;
+            uint32_t slot = sw.n_xf;
+            for (size_t o = sw.op_begin; o < prog.ops.size(); o++) {
+                DOpHdr& h = prog.ops[o].h;
+                if (h.type != DOP_PHASETAB || h.s0 == 0) continue;
+                const size_t src = h.coef;
+                for (size_t i = 0; i < 16u * h.s0; i++) prog.coef.push_back(prog.coef[src + i]);
+                h.gmask = slot;
+                slot += 16u * h.s0;
+            }
+            sw.n_xf = slot;
+        }
+    }
     sw.op_end = uint32_t(prog.ops.size());
     prog.n_grad += n_grad_sweep;
     // The coefficients a tile reads while it runs the sweep go through the
@@ -988,14 +1090,32 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     // same tile layout (one more HBM pass each, far fewer cache misses).
     // Only sweeps of many tiles are split: with few workgroups in flight
     // the cache holds their coefficients anyway.
+    auto dense_width = [&](uint32_t o0, uint32_t o1) {
+        uint32_t w = 0;
+        for (uint32_t o = o0; o < o1; o++)
+            if (prog.ops[o].h.type == DOP_DENSE) w = std::max(w, uint32_t(__builtin_popcount(prog.ops[o].h.smask)));
+        return w;
+    };
+    sw.max_dense = dense_width(sw.op_begin, sw.op_end);
     if (prog.n_states == 1 && opt.coef_budget > 0 && sw.n_tiles >= 512) {
+        // bytes a tile reads through the scalar cache per phase: every op
+        // record (DOp stride) and the coefficients the op loads
         auto phase_bytes = [&](const PhaseInfo& ph) {
             long b = 0;
             for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
-                const DOpHdr& h = prog.ops[o].h;
-                if (h.type == DOP_DENSE) b += 16L << (2 * __builtin_popcount(h.smask));
-                else if (h.type == DOP_PHASESET) b += 32L * (h.s0 + h.s1);
-                else b += 40;
+                const DOp& d = prog.ops[o];
+                const DOpHdr& h = d.h;
+                b += long(sizeof(DOp));
+                switch (h.type) {
+                case DOP_DENSE: b += 16L << (2 * __builtin_popcount(h.smask)); break;
+                case DOP_PHASESET: b += 32L * (h.s0 + h.s1); break;
+                case DOP_PHASETAB: break;   // tables read with vector loads
+                case DOP_RTAB: b += 16L * kRegs; break;   // one variant per tile
+                case DOP_DIAG: b += 16L << d.nq; break;
+                case DOP_CDENSE: b += 64; break;
+                case DOP_CPHASE: b += 16; break;
+                default: break;
+                }
             }
             return b;
         };
@@ -1009,6 +1129,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                 part.phase_end = p;
                 part.op_begin = prog.phases[p0].op_begin;
                 part.op_end = prog.phases[p - 1].op_end;
+                part.max_dense = dense_width(part.op_begin, part.op_end);
                 prog.steps.push_back(Step{Step::SWEEP, uint32_t(prog.sweeps.size())});
                 prog.sweeps.push_back(part);
                 p0 = p;
@@ -1204,8 +1325,12 @@ std::string describe_program(const Program& prog)
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                     const DOp& d = prog.ops[o];
                     static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS", "CD",
-                                               "L", "L3", "T"};
+                                               "L", "L3", "T", "PT"};
                     ss << " " << nm[d.h.type];
+                    if (d.h.type == DOP_PHASETAB) {
+                        ss << d.h.s0;
+                        if (d.h.s1) ss << "x" << d.h.smask;
+                    }
                     if (d.h.type == DOP_PHASESET) {
                         ss << d.h.s0;
                         if (d.h.s1) ss << "u" << d.h.s1;

@@ -15,9 +15,11 @@ struct PlanOptions {
     int low_qubits = 4;        // c: physical qubits 0..c-1 in every tile
     long sweep_budget = 0;     // per-amplitude FMA budget of one sweep (0 = none)
     long op_overhead = 2;      // fusion cost model: per-op cost on top of its FMAs (per amplitude)
-    long coef_budget = 20000;  // split a sweep whose op coefficients exceed this many bytes (0 = never)
+    long coef_budget = 16000;  // split a sweep whose op records and coefficients exceed this many bytes
+                               // (0 = never)
     bool rotations = true;     // one-state phases: one-qubit unitaries as lifting rotations between
                                // merged register tables where the cost model prefers it
+    int merge_qubits = 2;      // widest dense block the register-run merging may form (1..4)
 };
 
 // One step of a program: a tiled sweep or a standalone full-state op.

@@ -43,7 +43,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
           5: "ERR_DEVICE", 6: "ERR_OOM", 7: "ERR_UNSUPPORTED"}
 
 (OPT_FUSION, OPT_TILE_QUBITS, OPT_LOW_QUBITS, OPT_SWEEP_FLOPS, OPT_PROFILE, OPT_PIPELINE, OPT_OP_COST,
- OPT_COEF_BUDGET, OPT_SPECIALIZE) = range(9)
+ OPT_COEF_BUDGET, OPT_SPECIALIZE, OPT_MERGE_QUBITS) = range(10)
 
 # Every exported entry point (include/svmi.h, include/svmi_debug.h).
 EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state", "svmi_set_random_state",
@@ -52,7 +52,7 @@ EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version", "svmi_qir_run", "svmi_qasm_parse",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
            "svmi_debug_plan_batch", "svmi_debug_plan_adjoint", "svmi_debug_emulate_qir", "svmi_debug_plan_qir",
-           "svmi_debug_jit", "svmi_dm_qir_run", "svmi_debug_emulate_dm_qir"]
+           "svmi_debug_jit", "svmi_dm_qir_run", "svmi_debug_emulate_dm_qir", "svmi_debug_step_times", "svmi_debug_fp64_rates"]
 
 
 class SvmiError(RuntimeError):
@@ -141,6 +141,9 @@ def lib():
         L.svmi_debug_plan_qir.argtypes = [ctypes.c_int, _vp, _sz, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                           ctypes.c_char_p, _sz]
+        L.svmi_debug_step_times.argtypes = [_vp, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)]
+        L.svmi_debug_fp64_rates.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]
         for name in EXPORTS:
             f = getattr(L, name)
             if name not in ("svmi_num_qubits", "svmi_last_error", "svmi_abi_version"):
@@ -314,6 +317,14 @@ class Engine:
         _check(lib().svmi_get_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
 
+    def step_times(self):
+        """Per-step device times (ms) of the last program run with profiling on."""
+        n = ctypes.c_uint64(0)
+        _check(lib().svmi_debug_step_times(self._h, None, 0, ctypes.byref(n)))
+        ms = np.zeros(max(n.value, 1))
+        _check(lib().svmi_debug_step_times(self._h, _ptr(ms), n.value, ctypes.byref(n)))
+        return ms[:n.value]
+
 
 def _splitmix64(z):
     z = (z + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -380,6 +391,13 @@ def batch_adjoint(items, device=0, _emulate=False):
 
 
 # ---- host checks of the planner (no device; see include/svmi_debug.h) ----
+
+def fp64_rates(waves_per_simd, device=0):
+    """(vector FMA, matrix-core) FP64 TFLOP/s measured on the device."""
+    v, m = ctypes.c_double(), ctypes.c_double()
+    _check(lib().svmi_debug_fp64_rates(int(device), int(waves_per_simd), ctypes.byref(v), ctypes.byref(m)))
+    return v.value, m.value
+
 
 def plan_batch(items):
     """{levels, launches, tasks, state_bytes} of the batch plan."""

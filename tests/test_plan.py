@@ -148,11 +148,13 @@ def test_batch_rejects_five_qubit_gates():
 @pytest.mark.parametrize("n,tile,low", [(16, 6, 2), (18, 8, 4), (17, 12, 4)])
 def test_phase_factors_match_oracle(n, tile, low):
     # QFT-pattern circuits put many controlled phases on qubits outside the
-    # tile; their product is formed once per tile (PS..u.. ops) -- check the
-    # plan really uses that and that the result is unchanged
+    # tile; their product is formed once per tile from nibble tables over the
+    # external bits (PT..x.. ops), the ones on group bits from tables over the
+    # group index -- check the plan really uses that and that the result is
+    # unchanged
     gates = W.qft(n=n)
     _, desc = svmi.plan(n, gates, tile_qubits=tile, low_qubits=low)
-    assert "u" in "".join(tok for tok in desc.split() if tok.startswith("PS"))
+    assert "x" in "".join(tok for tok in desc.split() if tok.startswith("PT"))
     st = W.random_state(n, seed=n + 40)
     assert _err(svmi.emulate(n, gates, st, tile_qubits=tile, low_qubits=low), O.run(n, gates, st)) < 1e-12
 
