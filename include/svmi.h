@@ -161,6 +161,19 @@ svmi_status svmi_measure(svmi_handle h, const double* uniforms, size_t shots,
 svmi_status svmi_expval(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms,
                         double* out);
 
+/* ---- the same expectation without any device allocation (no reference
+ *      counterpart): for callers that cannot allocate device memory (code
+ *      under HIP graph capture, tight serving loops, harnesses that forbid
+ *      hipMalloc).  Same result as svmi_expval (same kernels, same summation
+ *      order).  The Pauli tables and block partials live in a 1 MiB device
+ *      array that the library holds on every GPU it runs on until the process
+ *      ends; at most 16384 terms (SVMI_ERR_ARG beyond).  Calls on one GPU
+ *      share that array: calls that overlap in time on the same GPU (from
+ *      several threads or handles) race and must be serialised by the
+ *      caller. */
+svmi_status svmi_expval_static(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms,
+                               double* out);
+
 /* ---- adjoint differentiation (no reference counterpart).  Applies the
  *      uploaded circuit to the current state psi0 (the state afterwards is
  *      U psi0, as after svmi_run), returns E = <H> and

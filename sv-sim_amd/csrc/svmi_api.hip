@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include <cmath>
 #include <cstring>
@@ -48,6 +49,35 @@ svmi_status fail(svmi_status st, const std::string& msg)
     } while (0)
 
 constexpr int kMaxQubits = 40;
+
+// Device scratch of svmi_expval_static: a file-scope array that every GPU
+// the library runs on holds until the process ends.  Host code never passes
+// `scratch` itself to a kernel or copy (on the host it is only a stand-in for
+// the symbol); it uses the device address hipGetSymbolAddress returns for
+// the current device, looked up once per device by svmi_create.
+constexpr size_t kStaticScratchBytes = size_t(1) << 20;
+constexpr size_t kStaticMaxTerms = 16384;   // Pauli terms of one svmi_expval_static call
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_static_scratch[kMaxDevices] = {};   // per-device address of `scratch`
+std::mutex g_static_scratch_mu;
+
+// Device address of `scratch` on `device` (which must be current).
+svmi_status static_scratch_ptr(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(SVMI_ERR_DEVICE, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_static_scratch_mu);
+    if (!g_static_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(SVMI_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_static_scratch[device] = ptr;
+    }
+    *out = g_static_scratch[device];
+    return SVMI_OK;
+}
 
 // Growable device buffer.
 struct DevBuf {
@@ -641,6 +671,7 @@ struct svmi_sim {
     Program adj_prog;
     DeviceProgram adj_dprog;
     DevBuf adj_gsum;
+    void* static_scratch = nullptr;   // this device's address of the file-scope scratch array
     // QIR runs: program, device copy, measurement results; the P(1) of the
     // latest measurement (0 after create / reset_state)
     Program qir_prog;
@@ -836,6 +867,11 @@ svmi_status svmi_create(int n_qubits, int device, svmi_handle* out)
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&h->ev0);
     if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e == hipSuccess && static_scratch_ptr(device, &h->static_scratch) != SVMI_OK) {
+        const std::string msg = g_err;
+        svmi_destroy(h);
+        return fail(SVMI_ERR_DEVICE, "create: " + msg);
+    }
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->sv), h->dim * sizeof(c64));
     if (e == hipSuccess) e = launch_fill_basis(h->stream, h->sv, h->dim);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1199,6 +1235,42 @@ svmi_status svmi_expval(svmi_handle h, const svmi_pauli_term* terms, size_t n_te
     if (!out) return fail(SVMI_ERR_ARG, "null output");
     SVMI_TRY(check_terms(terms, n_terms, h->n));
     return pauli_device(h, h->sv, nullptr, terms, n_terms, out);
+}
+
+svmi_status svmi_expval_static(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms, double* out)
+{
+    SVMI_TRY(use_device(h));
+    if (!out) return fail(SVMI_ERR_ARG, "null output");
+    SVMI_TRY(check_terms(terms, n_terms, h->n));
+    if (n_terms > kStaticMaxTerms)
+        return fail(SVMI_ERR_ARG, "svmi_expval_static takes at most " + std::to_string(kStaticMaxTerms) +
+                                      " Pauli terms (got " + std::to_string(n_terms) + "); use svmi_expval");
+    std::vector<PauliGroup> groups;
+    std::vector<PauliTerm> pt;
+    group_terms(terms, n_terms, groups, pt);
+    // scratch layout: groups | terms | block partials, each 256-B aligned
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const uint32_t grid = pauli_grid(h->dim);
+    const size_t og = 0, ot = up(groups.size() * sizeof(PauliGroup)), op = ot + up(pt.size() * sizeof(PauliTerm));
+    if (op + size_t(grid) * sizeof(double) > kStaticScratchBytes)
+        return fail(SVMI_ERR_ARG, "svmi_expval_static: tables and partials exceed the static scratch");
+    unsigned char* base = static_cast<unsigned char*>(h->static_scratch);
+    PauliGroup* dg = reinterpret_cast<PauliGroup*>(base + og);
+    PauliTerm* dt = reinterpret_cast<PauliTerm*>(base + ot);
+    double* dp = reinterpret_cast<double*>(base + op);
+    if (!groups.empty()) {
+        SVMI_HIP(hipMemcpyAsync(dg, groups.data(), groups.size() * sizeof(PauliGroup), hipMemcpyHostToDevice,
+                                h->stream));
+        SVMI_HIP(hipMemcpyAsync(dt, pt.data(), pt.size() * sizeof(PauliTerm), hipMemcpyHostToDevice, h->stream));
+    }
+    SVMI_HIP(launch_pauli(h->stream, h->sv, nullptr, h->dim, dg, uint32_t(groups.size()), dt, dp));
+    std::vector<double> part(grid);
+    SVMI_HIP(hipMemcpyAsync(part.data(), dp, grid * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    SVMI_HIP(hipStreamSynchronize(h->stream));
+    double sum = 0.0;
+    for (double v : part) sum += v;
+    *out = sum;
+    return SVMI_OK;
 }
 
 svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_terms, const int32_t* param_slots,

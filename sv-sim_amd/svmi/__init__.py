@@ -52,7 +52,7 @@ EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state",
            "svmi_set_option", "svmi_get_stats", "svmi_last_error", "svmi_abi_version", "svmi_qir_run", "svmi_qasm_parse",
            "svmi_debug_emulate", "svmi_debug_emulate_adjoint", "svmi_debug_emulate_batch", "svmi_debug_plan",
            "svmi_debug_plan_batch", "svmi_debug_plan_adjoint", "svmi_debug_emulate_qir", "svmi_debug_plan_qir",
-           "svmi_debug_jit", "svmi_dm_qir_run", "svmi_debug_emulate_dm_qir", "svmi_debug_step_times", "svmi_debug_fp64_rates"]
+           "svmi_debug_jit", "svmi_dm_qir_run", "svmi_debug_emulate_dm_qir", "svmi_debug_step_times", "svmi_debug_fp64_rates", "svmi_expval_static"]
 
 
 class SvmiError(RuntimeError):
@@ -115,6 +115,7 @@ def lib():
         L.svmi_run.argtypes = [_vp]
         L.svmi_measure.argtypes = [_vp, _vp, _sz, _vp, _vp]
         L.svmi_expval.argtypes = [_vp, _vp, _sz, _vp]
+        L.svmi_expval_static.argtypes = [_vp, _vp, _sz, _vp]
         L.svmi_adjoint.argtypes = [_vp, _vp, _sz, _vp, _sz, _vp, _vp]
         L.svmi_batch_adjoint.argtypes = [ctypes.c_int, _vp, _sz, _vp, _vp]
         L.svmi_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int64]
