@@ -86,8 +86,11 @@ def bench_u3_brickwall(n, steps, warmup):
     ts = time.perf_counter() - t1
     passes = e.stats()["sweeps"]
     e.close()
+    # FP64 multiply-adds per amplitude of the planned program (planner summary)
+    _, desc = svmi.plan(n, gates)
+    fma = float(desc.split("fma_per_amp=")[1].split()[0])
     return {"gates": len(gates), "dt": dt, "sweep_ms": sweep_ms, "sweep_n": sweep_n, "other_ms": other_ms,
-            "passes": passes, "sample_s": ts, "total_prob": total}
+            "passes": passes, "sample_s": ts, "total_prob": total, "fma_per_amp": fma}
 
 
 def bench_qft(n, steps, seed=2033):
@@ -714,7 +717,7 @@ def main():
         "config": {"workload": f"u3_brickwall n={a.qubits} layers=20 (890 gates, FP64, state in HBM)",
                    "qubits": a.qubits, "gates": r["gates"], "passes_per_step": r["passes"],
                    "sweep_GBps": r["passes"] * bytes_per_sweep / (r["dt"] / a.steps) / 1e9,
-                   "sampling_shots_per_s": 4096 / r["sample_s"]},
+                   "sampling_shots_per_s": 4096 / r["sample_s"], "fma_per_amp": r["fma_per_amp"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(a.qubits)},
         "cpu_baseline": cpu,

@@ -113,10 +113,20 @@ struct DeviceProgram {
         if (!v.empty()) SVMI_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
         return SVMI_OK;
     }
+    // op arrays end with one NOP record: the sweep kernels read the header
+    // after each phase's last op
+    static svmi_status put_ops(DevBuf& b, const std::vector<DOp>& v, hipStream_t st)
+    {
+        static const DOp nop = {};
+        SVMI_TRY(b.ensure((v.size() + 1) * sizeof(DOp)));
+        if (!v.empty()) SVMI_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(DOp), hipMemcpyHostToDevice, st));
+        SVMI_HIP(hipMemcpyAsync(static_cast<DOp*>(b.p) + v.size(), &nop, sizeof(DOp), hipMemcpyHostToDevice, st));
+        return SVMI_OK;
+    }
     svmi_status upload(const Program& p, hipStream_t st)
     {
         SVMI_TRY(put(phases, p.phases, st));
-        SVMI_TRY(put(ops, p.ops, st));
+        SVMI_TRY(put_ops(ops, p.ops, st));
         SVMI_TRY(put(coef, p.coef, st));
         SVMI_TRY(put(hitab, p.hitab, st));
         SVMI_HIP(hipStreamSynchronize(st));
@@ -1330,25 +1340,82 @@ svmi_status svmi_adjoint(svmi_handle h, const svmi_pauli_term* terms, size_t n_t
 // Batched expectation + adjoint gradients: every circuit is planned on the
 // host (in parallel), the programs are concatenated into one device program
 // and one launch per dependency level runs the tasks of every circuit at
-// that level (no host synchronisation between the levels).
+// that level (no host synchronisation between the levels).  The circuits go
+// in chunks, costliest first (gates x 2^n): while the GPU runs chunk c the
+// host plans chunk c + 1, so planning hides behind the large circuits' device
+// time instead of preceding it.
 svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t n_items, double* expvals,
                                double* grads)
 {
     if (n_items == 0) return SVMI_OK;
-    BatchPlan bp;
-    SVMI_TRY(build_batch(items, n_items, expvals, grads, bp));
+    if (!items || !expvals) return fail(SVMI_ERR_ARG, "null batch arrays");
     int ndev = 0;
     SVMI_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(SVMI_ERR_DEVICE, "device " + std::to_string(device) + " not present");
     SVMI_HIP(hipSetDevice(device));
     SVMI_TRY(ensure_kernels());
 
-    DevBuf dstate, dsteps, dcircs, dsweeps, dphases, dops, dcoef, dhitab, dgroups, dterms, dpart, dsegs, dout;
-    auto release_all = [&]() {
-        for (DevBuf* b : {&dstate, &dsteps, &dcircs, &dsweeps, &dphases, &dops, &dcoef, &dhitab, &dgroups, &dterms,
-                          &dpart, &dsegs, &dout})
-            b->release();
+    // chunks by cumulative cost: the first quarter of the work, up to 90 %,
+    // the rest (many small circuits: mostly planning, little device time)
+    std::vector<size_t> order(n_items);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<double> cost(n_items);
+    double total = 0.0;
+    for (size_t i = 0; i < n_items; i++) {
+        const int n = std::max(int(items[i].n_qubits), kRegBits);
+        cost[i] = double(items[i].n_gates + 1) * std::ldexp(1.0, std::min(n, 60));
+        total += cost[i];
+    }
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<size_t> bounds{0};
+    {
+        const double cut[] = {0.25, 0.9};
+        double acc = 0.0;
+        int ci = 0;
+        for (size_t j = 0; j < n_items; j++) {
+            acc += cost[order[j]];
+            if (ci < 2 && acc >= cut[ci] * total && j + 1 < n_items) {
+                bounds.push_back(j + 1);
+                while (ci < 2 && acc >= cut[ci] * total) ci++;
+            }
+        }
+        bounds.push_back(n_items);
+    }
+    const size_t n_chunks = bounds.size() - 1;
+    // parameter offset of each item in grads, state offset in the one state buffer
+    std::vector<size_t> goff(n_items + 1, 0);
+    std::vector<uint64_t> soff(n_items + 1, 0);
+    for (size_t i = 0; i < n_items; i++) {
+        goff[i + 1] = goff[i] + items[i].n_params;
+        const uint64_t dim = 1ull << std::max(int(items[i].n_qubits), kRegBits);
+        soff[i + 1] = soff[i] + (((2 * dim * sizeof(c64)) + 255) & ~uint64_t(255));
+    }
+    if (goff[n_items] && !grads) return fail(SVMI_ERR_ARG, "null gradient array");
+
+    struct Chunk {
+        std::vector<size_t> idx;                  // original item indices
+        std::vector<svmi_batch_item> items;
+        BatchPlan bp;
+        DevBuf dsteps, dcircs, dsweeps, dphases, dops, dcoef, dhitab, dgroups, dterms, dpart, dsegs, dout;
+        void release()
+        {
+            for (DevBuf* b : {&dsteps, &dcircs, &dsweeps, &dphases, &dops, &dcoef, &dhitab, &dgroups, &dterms, &dpart,
+                              &dsegs, &dout})
+                b->release();
+        }
     };
+    std::vector<Chunk> ch(n_chunks);
+    auto plan_chunk = [&](size_t c) -> svmi_status {
+        Chunk& k = ch[c];
+        for (size_t j = bounds[c]; j < bounds[c + 1]; j++) {
+            k.idx.push_back(order[j]);
+            k.items.push_back(items[order[j]]);
+        }
+        std::vector<double> dummy(1);
+        return build_batch(k.items.data(), k.items.size(), dummy.data(), dummy.data(), k.bp);
+    };
+
+    DevBuf dstate;
     hipStream_t st = nullptr;
     svmi_status rc = SVMI_OK;
     auto put = [&](DevBuf& b, const void* src, size_t bytes) -> svmi_status {
@@ -1356,71 +1423,90 @@ svmi_status svmi_batch_adjoint(int device, const svmi_batch_item* items, size_t 
         if (bytes) SVMI_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
         return SVMI_OK;
     };
+    auto launch_chunk = [&](size_t c) -> svmi_status {
+        Chunk& k = ch[c];
+        BatchPlan& bp = k.bp;
+        std::vector<BatchCircuit> circs = bp.circs;
+        for (size_t j = 0; j < circs.size(); j++) {
+            circs[j].phi = reinterpret_cast<c64*>(static_cast<char*>(dstate.p) + soff[k.idx[j]]);
+            circs[j].lam = circs[j].phi + (1ull << circs[j].n);
+        }
+        for (size_t j : bp.need_fill)
+            SVMI_HIP(launch_fill_basis(st, circs[j].phi, 1ull << circs[j].n));
+        SVMI_TRY(put(k.dsteps, bp.steps.data(), bp.steps.size() * sizeof(BatchStep)));
+        SVMI_TRY(put(k.dcircs, circs.data(), circs.size() * sizeof(BatchCircuit)));
+        SVMI_TRY(put(k.dsweeps, bp.sweeps.data(), bp.sweeps.size() * sizeof(SweepInfo)));
+        SVMI_TRY(put(k.dphases, bp.phases.data(), bp.phases.size() * sizeof(PhaseInfo)));
+        bp.ops.push_back(DOp{});   // a NOP past the end: the kernels read one header ahead
+        SVMI_TRY(put(k.dops, bp.ops.data(), bp.ops.size() * sizeof(DOp)));
+        SVMI_TRY(put(k.dcoef, bp.coef.data(), bp.coef.size() * sizeof(c64)));
+        SVMI_TRY(put(k.dhitab, bp.hitab.data(), bp.hitab.size() * sizeof(uint64_t)));
+        SVMI_TRY(put(k.dgroups, bp.groups.data(), bp.groups.size() * sizeof(PauliGroup)));
+        SVMI_TRY(put(k.dterms, bp.terms.data(), bp.terms.size() * sizeof(PauliTerm)));
+        SVMI_TRY(k.dpart.ensure(std::max<size_t>((bp.n_grad_partials + bp.n_e_partials) * sizeof(double), 16)));
+        SVMI_TRY(put(k.dsegs, bp.segs.data(), bp.segs.size() * sizeof(SumSegment)));
+        SVMI_TRY(k.dout.ensure((bp.n_slots + k.items.size()) * sizeof(double)));
+        BatchArgs a;
+        a.steps = k.dsteps.as<BatchStep>();
+        a.circuits = k.dcircs.as<BatchCircuit>();
+        a.sweeps = k.dsweeps.as<SweepInfo>();
+        a.phases = k.dphases.as<PhaseInfo>();
+        a.ops = k.dops.as<DOp>();
+        a.coef = k.dcoef.as<c64>();
+        a.hitab = k.dhitab.as<uint64_t>();
+        a.groups = k.dgroups.as<PauliGroup>();
+        a.terms = k.dterms.as<PauliTerm>();
+        a.grad_partials = k.dpart.as<double>();
+        a.e_partials = k.dpart.as<double>() + bp.n_grad_partials;
+        for (const BatchLaunch& L : bp.launches) SVMI_HIP(launch_batch(st, a, L));
+        SVMI_HIP(launch_sum_segments(st, k.dpart.as<double>(), k.dsegs.as<SumSegment>(), bp.segs.size(),
+                                     k.dout.as<double>()));
+        return SVMI_OK;
+    };
     do {
+        if ((rc = plan_chunk(0)) != SVMI_OK) break;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
             rc = fail(SVMI_ERR_DEVICE, "batch: stream creation failed");
             break;
         }
-        if ((rc = dstate.ensure(bp.state_bytes)) != SVMI_OK) break;
-        std::vector<BatchCircuit> circs = bp.circs;
-        for (size_t i = 0; i < circs.size(); i++) {
-            circs[i].phi = reinterpret_cast<c64*>(static_cast<char*>(dstate.p) + bp.state_off[i]);
-            circs[i].lam = circs[i].phi + (1ull << circs[i].n);
-        }
-        for (size_t i : bp.need_fill) {
-            if (launch_fill_basis(st, circs[i].phi, 1ull << circs[i].n) != hipSuccess) {
-                rc = fail(SVMI_ERR_DEVICE, "batch: fill failed");
-                break;
-            }
+        if ((rc = dstate.ensure(std::max<uint64_t>(soff[n_items], 16))) != SVMI_OK) break;
+        for (size_t c = 0; c < n_chunks && rc == SVMI_OK; c++) {
+            rc = launch_chunk(c);
+            // the next chunk's plans while this one runs
+            if (rc == SVMI_OK && c + 1 < n_chunks) rc = plan_chunk(c + 1);
         }
         if (rc != SVMI_OK) break;
-        if ((rc = put(dsteps, bp.steps.data(), bp.steps.size() * sizeof(BatchStep))) != SVMI_OK) break;
-        if ((rc = put(dcircs, circs.data(), circs.size() * sizeof(BatchCircuit))) != SVMI_OK) break;
-        if ((rc = put(dsweeps, bp.sweeps.data(), bp.sweeps.size() * sizeof(SweepInfo))) != SVMI_OK) break;
-        if ((rc = put(dphases, bp.phases.data(), bp.phases.size() * sizeof(PhaseInfo))) != SVMI_OK) break;
-        if ((rc = put(dops, bp.ops.data(), bp.ops.size() * sizeof(DOp))) != SVMI_OK) break;
-        if ((rc = put(dcoef, bp.coef.data(), bp.coef.size() * sizeof(c64))) != SVMI_OK) break;
-        if ((rc = put(dhitab, bp.hitab.data(), bp.hitab.size() * sizeof(uint64_t))) != SVMI_OK) break;
-        if ((rc = put(dgroups, bp.groups.data(), bp.groups.size() * sizeof(PauliGroup))) != SVMI_OK) break;
-        if ((rc = put(dterms, bp.terms.data(), bp.terms.size() * sizeof(PauliTerm))) != SVMI_OK) break;
-        if ((rc = dpart.ensure((bp.n_grad_partials + bp.n_e_partials) * sizeof(double))) != SVMI_OK) break;
-        if ((rc = put(dsegs, bp.segs.data(), bp.segs.size() * sizeof(SumSegment))) != SVMI_OK) break;
-        if ((rc = dout.ensure((bp.n_slots + n_items) * sizeof(double))) != SVMI_OK) break;
-        BatchArgs a;
-        a.steps = dsteps.as<BatchStep>();
-        a.circuits = dcircs.as<BatchCircuit>();
-        a.sweeps = dsweeps.as<SweepInfo>();
-        a.phases = dphases.as<PhaseInfo>();
-        a.ops = dops.as<DOp>();
-        a.coef = dcoef.as<c64>();
-        a.hitab = dhitab.as<uint64_t>();
-        a.groups = dgroups.as<PauliGroup>();
-        a.terms = dterms.as<PauliTerm>();
-        a.grad_partials = dpart.as<double>();
-        a.e_partials = dpart.as<double>() + bp.n_grad_partials;
-        hipError_t e = hipSuccess;
-        for (size_t l = 0; l < bp.launches.size() && e == hipSuccess; l++) e = launch_batch(st, a, bp.launches[l]);
-        if (e == hipSuccess)
-            e = launch_sum_segments(st, dpart.as<double>(), dsegs.as<SumSegment>(), bp.segs.size(), dout.as<double>());
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        const hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
             rc = fail(SVMI_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
             break;
         }
-        std::vector<double> res(bp.n_slots + n_items);
-        e = hipMemcpy(res.data(), dout.p, res.size() * sizeof(double), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            rc = fail(SVMI_ERR_DEVICE, std::string("batch results: ") + hipGetErrorString(e));
-            break;
+        for (size_t c = 0; c < n_chunks && rc == SVMI_OK; c++) {
+            Chunk& k = ch[c];
+            const size_t m = k.items.size();
+            std::vector<double> res(k.bp.n_slots + m);
+            const hipError_t e2 = hipMemcpy(res.data(), k.dout.p, res.size() * sizeof(double), hipMemcpyDeviceToHost);
+            if (e2 != hipSuccess) {
+                rc = fail(SVMI_ERR_DEVICE, std::string("batch results: ") + hipGetErrorString(e2));
+                break;
+            }
+            for (size_t j = 0; j < m; j++) {
+                const size_t i = k.idx[j];
+                expvals[i] = res[k.bp.n_slots + j];
+                double* g = grads ? grads + goff[i] : nullptr;
+                for (size_t p = 0; p < items[i].n_params; p++) g[p] = 0.0;
+                const Program& ad = k.bp.adj[j];
+                for (uint32_t s2 = 0; s2 < ad.n_grad; s2++)
+                    g[ad.grad_param[s2]] += 2.0 * res[k.bp.circs[j].slot_begin + s2];
+            }
         }
-        const std::vector<double> gs(res.begin(), res.begin() + bp.n_slots), eo(res.begin() + bp.n_slots, res.end());
-        finish_batch(bp, items, n_items, gs, eo, expvals, grads);
     } while (false);
     if (st) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
-    release_all();
+    for (Chunk& k : ch) k.release();
+    dstate.release();
     return rc;
 }
 

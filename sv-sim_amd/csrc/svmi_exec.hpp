@@ -247,13 +247,7 @@ SVMI_HD inline void apply_phasetab(const DOpHdr& h, c64 (&a)[N], const c64* __re
 }
 
 // The tile base's bits outside the tile, packed in ascending qubit order.
-SVMI_HD inline uint64_t ext_index(const SweepInfo& sw, uint64_t base)
-{
-    uint64_t x = 0;
-    int j = 0;
-    for (uint64_t m = sw.extmask; m; m &= m - 1, j++) x |= ((base >> __builtin_ctzll(m)) & 1ull) << j;
-    return x;
-}
+SVMI_HD inline uint64_t ext_index(const SweepInfo& sw, uint64_t base) { return extract_bits(base, sw.extmask); }
 
 // Factor slot of a DOP_PHASETAB op for the tile with external bits x.
 SVMI_HD inline c64 phasetab_factor(const DOpHdr& h, const c64* __restrict__ coef, uint64_t x)
@@ -295,6 +289,26 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
                                   const c64* __restrict__ coef, uint32_t goff, uint32_t grp,
                                   const uint32_t (&off)[kRegs], uint64_t base, const c64* xf)
 {
+    // the op kinds of rotation/table phases first (one or two compares),
+    // then the rest
+    if (h.type == DOP_LIFT2) {
+        SVMI_FENCE(DOP_LIFT2 + 100);
+        dispatch_lift<false, N>(h, a);
+        SVMI_FENCE(DOP_LIFT2 + 100);
+        return;
+    }
+    if (h.type == DOP_RTAB) {
+        SVMI_FENCE(DOP_RTAB + 100);
+        apply_rtab<N>(a, coef + h.coef + 16u * rtab_variant(h.emask, base));
+        SVMI_FENCE(DOP_RTAB + 100);
+        return;
+    }
+    if (h.type == DOP_LIFT3) {
+        SVMI_FENCE(DOP_LIFT3 + 100);
+        dispatch_lift<true, N>(h, a);
+        SVMI_FENCE(DOP_LIFT3 + 100);
+        return;
+    }
     switch (h.type) {
         SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, (dispatch_dense<kMaxQ, N>(h.smask, a, coef + h.coef)))
         SVMI_CASE_ID(DOP_X, DOP_X + 100, dispatch_x<N>(h.s0, a))
@@ -305,9 +319,6 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag<N>(*op, a, coef, goff, off, base))
         SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset<N>(h, a, coef, goff, base, xf))
         SVMI_CASE_ID(DOP_CDENSE, DOP_CDENSE + 100, dispatch_cdense<N>(h, a, coef, goff, base))
-        SVMI_CASE_ID(DOP_LIFT2, DOP_LIFT2 + 100, (dispatch_lift<false, N>(h, a)))
-        SVMI_CASE_ID(DOP_LIFT3, DOP_LIFT3 + 100, (dispatch_lift<true, N>(h, a)))
-        SVMI_CASE_ID(DOP_RTAB, DOP_RTAB + 100, apply_rtab<N>(a, coef + h.coef + 16u * rtab_variant(h.emask, base)))
         SVMI_CASE_ID(DOP_PHASETAB, DOP_PHASETAB + 100, apply_phasetab<N>(h, a, coef, grp, xf))
     default: break;
     }
@@ -365,12 +376,8 @@ SVMI_HD inline uint32_t lds_slot(uint32_t l) { return l ^ ((l >> 4) & 15u); }
 // low bits on extmask & ~exthi, high bits on exthi (each ascending).
 SVMI_HD inline uint64_t tile_base(const SweepInfo& sw, uint64_t tile)
 {
-    uint64_t base = 0;
-    for (uint64_t m = sw.extmask & ~sw.exthi; m; m &= m - 1, tile >>= 1)
-        base |= (tile & 1ull) << __builtin_ctzll(m);
-    for (uint64_t m = sw.exthi; m; m &= m - 1, tile >>= 1)
-        base |= (tile & 1ull) << __builtin_ctzll(m);
-    return base;
+    const uint64_t lo = sw.extmask & ~sw.exthi;
+    return deposit_bits(tile, lo) | deposit_bits(tile >> __builtin_popcountll(lo), sw.exthi);
 }
 
 }  // namespace svmi

@@ -399,13 +399,38 @@ SVMI_HD inline void apply_lift(c64 (&a)[N], double u, double l)
         }
 }
 
-// Register table (DOP_RTAB): table variant of the tile at `base`.
-SVMI_HD inline uint32_t rtab_variant(uint64_t emask, uint64_t base)
+// Bit deposit / extract over the set bits of `mask` (ascending), one step
+// per run of consecutive set bits rather than per bit.  mask < 2^63.
+SVMI_HD inline uint64_t deposit_bits(uint64_t v, uint64_t mask)
 {
-    uint32_t v = 0, b = 0;
-    for (uint64_t m = emask; m; m &= m - 1, b++) v |= uint32_t((base >> __builtin_ctzll(m)) & 1u) << b;
-    return v;
+    uint64_t r = 0;
+    while (mask) {
+        const int p = __builtin_ctzll(mask);
+        const int len = __builtin_ctzll(~(mask >> p));
+        const uint64_t run = (uint64_t(1) << len) - 1u;
+        r |= (v & run) << p;
+        v >>= len;
+        mask &= ~(run << p);
+    }
+    return r;
 }
+SVMI_HD inline uint64_t extract_bits(uint64_t v, uint64_t mask)
+{
+    uint64_t r = 0;
+    int j = 0;
+    while (mask) {
+        const int p = __builtin_ctzll(mask);
+        const int len = __builtin_ctzll(~(mask >> p));
+        const uint64_t run = (uint64_t(1) << len) - 1u;
+        r |= ((v >> p) & run) << j;
+        j += len;
+        mask &= ~(run << p);
+    }
+    return r;
+}
+
+// Register table (DOP_RTAB): table variant of the tile at `base`.
+SVMI_HD inline uint32_t rtab_variant(uint64_t emask, uint64_t base) { return uint32_t(extract_bits(base, emask)); }
 
 template <int N = kRegs>
 SVMI_HD inline void apply_rtab(c64 (&a)[N], const c64* __restrict__ T)

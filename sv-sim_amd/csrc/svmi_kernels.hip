@@ -25,13 +25,12 @@ namespace svmi {
 // XOR-swizzled (lds_slot) at LDS index lo + l: lo = 0, or 2^k for lambda's
 // half of a two-state tile.
 
-// HBM (or |0..0> when init_zero) -> registers r[j] for tile `tile`.
+// HBM (or |0..0> when init_zero) -> registers r[j] for the tile at `base`.
 template <int J>
 __device__ __attribute__((always_inline)) inline void
 tile_fetch(c64 (&r)[J], const c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht,
-           uint64_t tile, uint32_t tid, uint32_t nt, bool init_zero)
+           uint64_t base, uint32_t tid, uint32_t nt, bool init_zero)
 {
-    const uint64_t base = tile_base(sw, tile);
     const uint32_t cmask = (1u << sw.c) - 1u;
 #pragma unroll
     for (int j = 0; j < J; j++) {
@@ -49,13 +48,12 @@ tile_put(const c64 (&r)[J], c64* t, uint32_t lo, uint32_t tid, uint32_t nt)
     for (int j = 0; j < J; j++) t[lds_slot(lo + tid + nt * j)] = r[j];
 }
 
-// LDS -> HBM for tile `tile`.
+// LDS -> HBM for the tile at `base`.
 template <int J>
 __device__ __attribute__((always_inline)) inline void
 tile_store(const c64* t, uint32_t lo, c64* __restrict__ s, const SweepInfo& sw, const uint64_t* __restrict__ ht,
-           uint64_t tile, uint32_t tid, uint32_t nt)
+           uint64_t base, uint32_t tid, uint32_t nt)
 {
-    const uint64_t base = tile_base(sw, tile);
     const uint32_t cmask = (1u << sw.c) - 1u;
 #pragma unroll
     for (int j = 0; j < J; j++) {
@@ -115,17 +113,19 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
 #pragma unroll
             for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(goff | off[r])];
             // op headers are fetched one op ahead so the scalar-load latency
-            // of the next op overlaps the arithmetic of the current one
-            DOpHdr nxt;
-            if (ph.op_begin < ph.op_end) nxt = ops[ph.op_begin].h;
-            for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
+            // of the next op overlaps the arithmetic of the current one (every
+            // op array ends with a NOP, so the read past a phase's last op
+            // stays inside it)
+            const DOp* __restrict__ op = ops + ph.op_begin;
+            DOpHdr nxt = op->h;
+            for (uint32_t o = ph.op_begin; o < ph.op_end; o++, op++) {
                 const DOpHdr h = nxt;
-                if (o + 1 < ph.op_end) nxt = ops[o + 1].h;
+                nxt = op[1].h;
                 if constexpr (NS == 2) {
                     if (h.type == DOP_GRAD_DENSE || h.type == DOP_GRAD_DIAG) {
                         // every gradient op of a sweep has its own slot and each
                         // wave runs it once: a plain store, no read
-                        double v = exec_grad_op<kMaxDenseQubitsAdj, kRegsAdj>(h, ops + o, a, coef, goff, off, base);
+                        double v = exec_grad_op<kMaxDenseQubitsAdj, kRegsAdj>(h, op, a, coef, goff, off, base);
                         if (nt >= 64) {
                             v = wave_sum_lane63(v);
                             if ((tid & 63u) == 63u) gacc[h.grad * nwaves + (tid >> 6)] = v;
@@ -137,7 +137,7 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
                         continue;
                     }
                 }
-                exec_state_op<kMaxQ>(h, ops + o, a, coef, goff, tid, off, base, xf);
+                exec_state_op<kMaxQ>(h, op, a, coef, goff, tid, off, base, xf);
             }
 #pragma unroll
             for (int r = 0; r < kRegs; r++) t0[lds_slot(goff | off[r])] = a[r];
@@ -189,8 +189,8 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     if (active) {
         // both states' loads are issued before either waits
         c64 r[J], r1[J];
-        tile_fetch(r, s0, sw, ht, tile, tid, nt, init_zero);
-        if constexpr (NS == 2) tile_fetch(r1, s1, sw, ht, tile, tid, nt, false);
+        tile_fetch(r, s0, sw, ht, base, tid, nt, init_zero);
+        if constexpr (NS == 2) tile_fetch(r1, s1, sw, ht, base, tid, nt, false);
         tile_put(r, t0, 0u, tid, nt);
         if constexpr (NS == 2) tile_put(r1, t0, tsize, tid, nt);
     }
@@ -200,8 +200,8 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     __syncthreads();
     tile_phases<NS, kIdle, kMaxQ>(t0, gacc, xf, sw, phases, ops, coef, base, tid, nt);
     if (active) {
-        tile_store<J>(t0, 0u, s0, sw, ht, tile, tid, nt);
-        if constexpr (NS == 2) tile_store<J>(t0, tsize, s1, sw, ht, tile, tid, nt);
+        tile_store<J>(t0, 0u, s0, sw, ht, base, tid, nt);
+        if constexpr (NS == 2) tile_store<J>(t0, tsize, s1, sw, ht, base, tid, nt);
     }
     // gradient partials are stored grad-major ([i][tile]) so the sum over
     // tiles of one gradient reads one contiguous segment
@@ -254,7 +254,7 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
     const uint64_t* ht = hitab + sw.hitab;
     uint64_t t = blockIdx.x;
     c64 nx[kRegs];
-    tile_fetch(nx, s0, sw, ht, tile_offset + t, tid, nt, false);
+    tile_fetch(nx, s0, sw, ht, tile_base(sw, tile_offset + t), tid, nt, false);
     for (;;) {
         const uint64_t base = tile_base(sw, tile_offset + t);
         tile_put(nx, t0, 0u, tid, nt);
@@ -262,9 +262,9 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
         __syncthreads();
         const uint64_t tn = t + gridDim.x;
         const bool more = tn < n_tiles;
-        if (more) tile_fetch(nx, s0, sw, ht, tile_offset + tn, tid, nt, false);
+        if (more) tile_fetch(nx, s0, sw, ht, tile_base(sw, tile_offset + tn), tid, nt, false);
         tile_phases<1, false, kRegBits>(t0, nullptr, xf, sw, phases, ops, coef, base, tid, nt);
-        tile_store<kRegs>(t0, 0u, s0, sw, ht, tile_offset + t, tid, nt);
+        tile_store<kRegs>(t0, 0u, s0, sw, ht, base, tid, nt);
         if (!more) break;
         __syncthreads();  // the store's LDS reads finish before the next put
         t = tn;

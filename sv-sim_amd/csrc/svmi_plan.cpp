@@ -1294,6 +1294,39 @@ void build_adjoint_program(const std::vector<AdjEntry>& entries, int n, const Pl
     schedule(sops, n, o, prog);
 }
 
+// FP64 multiply-adds per amplitude the program's state ops execute (a
+// complex multiply counted as 4, a real shear as 1 per component pair...):
+// the arithmetic the sweeps do per amplitude, summed over the program.
+double program_fma_per_amp(const Program& prog)
+{
+    double f = 0.0;
+    for (const Step& st : prog.steps) {
+        if (st.kind != Step::SWEEP) continue;
+        const SweepInfo& sw = prog.sweeps[st.index];
+        for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++)
+            for (uint32_t o = prog.phases[p].op_begin; o < prog.phases[p].op_end; o++) {
+                const DOpHdr& h = prog.ops[o].h;
+                const double sel = __builtin_popcount(h.rsel & 0xffffu) / 16.0;   // share of registers
+                switch (h.type) {
+                case DOP_DENSE: f += 4.0 * (1 << __builtin_popcount(h.smask)); break;
+                case DOP_LIFT2: f += 2.0; break;
+                case DOP_LIFT3: f += 3.0; break;
+                case DOP_RTAB:
+                case DOP_DIAG: f += 4.0; break;
+                case DOP_CPHASE:
+                case DOP_PHASESET:
+                case DOP_PHASETAB: f += 4.0 * sel; break;
+                case DOP_CDENSE: f += 8.0 * sel; break;
+                case DOP_SRN: f += 2.0; break;
+                case DOP_GRAD_DENSE: f += 8.0 * (1 << __builtin_popcount(h.smask)); break;
+                case DOP_GRAD_DIAG: f += 6.0; break;
+                default: break;
+                }
+            }
+    }
+    return f;
+}
+
 std::string describe_program(const Program& prog)
 {
     std::ostringstream ss;
@@ -1303,7 +1336,7 @@ std::string describe_program(const Program& prog)
         (st.kind == Step::SWEEP ? n_sweeps : st.kind == Step::MEASURE ? n_meas : n_stand)++;
     ss << "n=" << prog.n << " steps=" << prog.steps.size() << " sweeps=" << n_sweeps
        << " standalone=" << n_stand << " measure=" << n_meas << " phases=" << prog.phases.size() << " dops=" << prog.ops.size()
-       << " lowered=" << prog.n_lowered << "\n";
+       << " lowered=" << prog.n_lowered << " fma_per_amp=" << program_fma_per_amp(prog) << "\n";
     for (const Step& st : prog.steps) {
         if (st.kind == Step::SWEEP) {
             const SweepInfo& sw = prog.sweeps[st.index];

@@ -86,6 +86,9 @@ struct AdjEntry {
 void build_adjoint_program(const std::vector<AdjEntry>& entries, int n, const PlanOptions& opt,
                            Program& prog);
 
+// FP64 multiply-adds per amplitude of the program's state ops.
+double program_fma_per_amp(const Program& prog);
+
 // Human-readable summary (sweeps, phases, ops per sweep).
 std::string describe_program(const Program& prog);
 
