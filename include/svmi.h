@@ -110,8 +110,12 @@ typedef enum svmi_option {
     SVMI_OPT_SPECIALIZE = 8,  /* 1 = compile each uploaded circuit's sweeps into straight-line
                                  kernels with hipRTC at upload (run-time compilation, seconds
                                  per circuit); 0 = interpret the sweep programs (default)     */
-    SVMI_OPT_MERGE_QUBITS = 9 /* widest dense block (1..4 register qubits) that fusion may
+    SVMI_OPT_MERGE_QUBITS = 9, /* widest dense block (1..4 register qubits) that fusion may
                                  form from a run of gates inside one phase (default 2)       */
+    SVMI_OPT_PREFETCH = 10    /* 1 = one-state sweeps of 12-qubit tiles over large states run
+                                 persistent with two LDS tile buffers, the next tile streaming
+                                 in by LDS DMA during the current one's phases (one workgroup
+                                 per CU); 0 = one workgroup per tile, two per CU (default)   */
 } svmi_option;
 
 /* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,

@@ -478,6 +478,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
         const uint32_t sw_base = uint32_t(bp.sweeps.size());
         for (DOp d : pr.ops) {
             d.h.coef += coef_base;
+            if (d.h.type == DOP_ROT && d.h.s0) d.h.grad += coef_base;   // its table
             bp.ops.push_back(d);
         }
         for (PhaseInfo ph : pr.phases) {
@@ -670,6 +671,7 @@ struct svmi_sim {
     PlanOptions opt;
     bool profile = false;
     int pipeline = 0;
+    bool prefetch = false;     // SVMI_OPT_PREFETCH
     std::vector<hipEvent_t> prof_events;
     std::vector<double> step_ms;   // with profiling: per-step times of the last program run
     svmi_stats stats{};
@@ -758,7 +760,7 @@ svmi_status exec_program(svmi_handle h, const Program& prog, DeviceProgram& dp, 
             else
                 SVMI_HIP(launch_sweep(h->stream, prog.n_states, s0, s1, sw, dp.phases.as<PhaseInfo>(),
                                       dp.ops.as<DOp>(), dp.coef.as<c64>(), dp.hitab.as<uint64_t>(), partials,
-                                      h->pipeline));
+                                      h->pipeline, h->prefetch));
             if (partials && !combined)
                 SVMI_HIP(launch_sum_rows(h->stream, partials, sw.n_grad, sw.n_tiles, grad_sums + sw.grad_base));
         } else if (st.kind == Step::MEASURE) {
@@ -1541,6 +1543,7 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
         if (value < 0 || value > 64) return fail(SVMI_ERR_ARG, "pipeline must be in [0, 64]");
         h->pipeline = int(value);
         return SVMI_OK;
+    case SVMI_OPT_PREFETCH: h->prefetch = value != 0; return SVMI_OK;
     case SVMI_OPT_SPECIALIZE: h->specialize = value != 0; break;
     case SVMI_OPT_MERGE_QUBITS:
         if (value < 1 || value > kRegBits) return fail(SVMI_ERR_ARG, "merge qubits must be in [1, 4]");

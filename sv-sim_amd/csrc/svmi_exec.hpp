@@ -238,8 +238,15 @@ SVMI_HD inline void apply_phasetab(const DOpHdr& h, c64 (&a)[N], const c64* __re
 {
     c64 ph = c64{1.0, 0.0};
     if (h.s1) ph = xf[h.grad];
-    const c64* T = h.gmask == kLaneTabGlobal ? coef + h.coef : xf + h.gmask;
-    for (uint32_t j = 0; j < h.s0; j++) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
+    // two loops, so the LDS tables are read as LDS (a pointer that may be
+    // either would be read with flat loads, which also wait for global ones)
+    if (h.gmask == kLaneTabGlobal) {
+        const c64* T = coef + h.coef;
+        for (uint32_t j = 0; j < h.s0; j++) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
+    } else {
+        const c64* T = xf + h.gmask;
+        for (uint32_t j = 0; j < h.s0; j++) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
+    }
     const uint32_t rsel = h.rsel;
 #pragma unroll
     for (int r = 0; r < N; r++)
@@ -282,6 +289,34 @@ SVMI_HD inline c64 tile_factor(const DOpHdr& h, const c64* __restrict__ coef, ui
     return h.type == DOP_PHASETAB ? phasetab_factor(h, coef, x) : phase_factor(h, coef, base);
 }
 
+// DOP_ROT: optional register table, then the lifts of the slots in smask.
+// The four {u, l} pairs are read together (one scalar load) and each slot
+// is one uniform branch.
+template <int N>
+SVMI_HD inline void apply_rot(const DOpHdr& h, c64 (&a)[N], const c64* __restrict__ coef, uint64_t base)
+{
+    if (h.s0) apply_rtab<N>(a, coef + h.grad + 16u * rtab_variant(h.emask, base));
+    const c64* L = coef + h.coef;
+    const c64 f0 = L[0], f1 = L[1], f2 = L[2], f3 = L[3];
+#define SVMI_ROT_SLOT(S, F)                                              \
+    if ((h.smask >> S) & 1u) {                                           \
+        if ((h.s1 >> S) & 1u) {                                          \
+            SVMI_FENCE(320 + S);                                         \
+            apply_lift<S, true, N>(a, F.re, F.im);                       \
+            SVMI_FENCE(320 + S);                                         \
+        } else {                                                         \
+            SVMI_FENCE(330 + S);                                         \
+            apply_lift<S, false, N>(a, F.re, F.im);                      \
+            SVMI_FENCE(330 + S);                                         \
+        }                                                                \
+    }
+    SVMI_ROT_SLOT(0, f0)
+    SVMI_ROT_SLOT(1, f1)
+    SVMI_ROT_SLOT(2, f2)
+    SVMI_ROT_SLOT(3, f3)
+#undef SVMI_ROT_SLOT
+}
+
 // h = the op's header (prefetched), op = the op in memory (diagonal tail),
 // grp = the group index (goff with the register bits squeezed out).
 template <int kMaxQ = kRegBits, int N = kRegs>
@@ -291,6 +326,12 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
 {
     // the op kinds of rotation/table phases first (one or two compares),
     // then the rest
+    if (h.type == DOP_ROT) {
+        SVMI_FENCE(DOP_ROT + 100);
+        apply_rot<N>(h, a, coef, base);
+        SVMI_FENCE(DOP_ROT + 100);
+        return;
+    }
     if (h.type == DOP_LIFT2) {
         SVMI_FENCE(DOP_LIFT2 + 100);
         dispatch_lift<false, N>(h, a);

@@ -115,6 +115,10 @@ enum DOpType : uint16_t {
                          // outside the tile (ext_index); T_j, E_j are 16 c64 each at coef;
                          // gmask = the factor-area slot where the tile keeps the T_j, or
                          // kLaneTabGlobal to read them from coef
+    DOP_ROT = 16,        // a run of rotations in one dispatch: if s0, first the register
+                         // table coef + grad + 16*v (v as DOP_RTAB, from emask); then the
+                         // lifting steps on every slot of smask in ascending order, three
+                         // shears where s1 has the slot's bit, {u, l} = coef[h.coef + slot]
 };
 constexpr uint32_t kLaneTabGlobal = 0xffffffffu;
 constexpr int kMaxXfSlots = 1024;      // factor area per sweep: 16 KiB of LDS

@@ -64,6 +64,14 @@ void emit_state_op(std::ostringstream& o, const DOp& d, const PhaseInfo& ph)
     case DOP_RTAB:
         o << "        apply_rtab(a, coef + " << h.coef << "u + 16u * rtab_variant(" << h.emask << "ull, base));\n";
         return;
+    case DOP_ROT:
+        if (h.s0)
+            o << "        apply_rtab(a, coef + " << h.grad << "u + 16u * rtab_variant(" << h.emask << "ull, base));\n";
+        for (int sl = 0; sl < kRegBits; sl++)
+            if ((h.smask >> sl) & 1u)
+                o << "        apply_lift<" << sl << ", " << (((h.s1 >> sl) & 1u) ? "true" : "false") << ">(a, coef["
+                  << (h.coef + uint32_t(sl)) << "].re, coef[" << (h.coef + uint32_t(sl)) << "].im);\n";
+        return;
     case DOP_CDENSE:
         o << "        if (" << cond << ") apply_cdense<" << h.s0 << ">(a, coef + " << h.coef << ", " << h.rsel
           << "u);\n";

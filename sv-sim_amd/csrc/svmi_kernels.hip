@@ -271,6 +271,113 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
     }
 }
 
+// ---------------------------------------------- double-buffered sweeps
+//
+// gfx950's LDS DMA (global_load_lds_dwordx4) moves 16 bytes per lane from
+// global memory straight into LDS, lane i of a wave at M0 + 16 i, without
+// holding registers.  It is issued from inline asm: the compiler cannot tell
+// the two tile buffers apart and would wait for the prefetch before every
+// LDS read of the other buffer; the sweep waits for it explicitly (vmcnt,
+// which gfx9-family GPUs decrement in issue order for loads and stores
+// alike) before the barrier that publishes the buffer.
+__device__ __attribute__((always_inline)) inline uint32_t lds_address(const void* p)
+{
+    return uint32_t(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void*)p));
+}
+
+__device__ __attribute__((always_inline)) inline void lds_dma16(const c64* src, uint32_t lds_byte)
+{
+    asm volatile("s_mov_b32 m0, %1\n\t"
+                 "s_nop 0\n\t"
+                 "global_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(src), "s"(lds_byte)
+                 : "memory", "m0");
+}
+
+constexpr uint32_t kDmaTileQubits = 12;
+constexpr uint32_t kDmaThreads = 1u << (kDmaTileQubits - kRegBits);    // 256
+constexpr uint32_t kDmaWaves = kDmaThreads / 64;                        // 4
+constexpr uint32_t kDmaPerWave = (1u << kDmaTileQubits) / 64 / kDmaWaves; // 16 DMA per wave per tile
+
+// Issue the DMA of the tile at `base` into the buffer at LDS byte address
+// lds_buf: the j-th DMA of wave w fills slots [64 (4 j + w), +64); slot S
+// holds tile-local index lds_slot(S) (the swizzle is an involution), whose
+// global index is base | ht[l >> c] | (l & cmask).  All addresses are formed
+// before the first DMA, so no LDS read waits behind one.
+__device__ __attribute__((always_inline)) inline void
+tile_dma(const c64* __restrict__ s, const uint64_t* ht, uint32_t c, uint64_t base, uint32_t lds_buf,
+         uint32_t wave, uint32_t lane)
+{
+    const uint32_t cmask = (1u << c) - 1u;
+    const c64* src[kDmaPerWave];
+#pragma unroll
+    for (uint32_t j = 0; j < kDmaPerWave; j++) {
+        const uint32_t l = lds_slot((j * kDmaWaves + wave) * 64u + lane);
+        src[j] = s + (base | ht[l >> c] | (l & cmask));
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kDmaPerWave; j++)
+        lds_dma16(src[j], __builtin_amdgcn_readfirstlane(lds_buf + (j * kDmaWaves + wave) * 64u * 16u));
+}
+
+size_t dma_sweep_smem_bytes(const SweepInfo& sw)
+{
+    return (size_t(32) << kDmaTileQubits) + (size_t(8) << (kDmaTileQubits - sw.c)) + size_t(16) * sw.n_xf;
+}
+
+// One state, persistent, two tile buffers of 2^12 amplitudes: while the
+// phases of tile t run in one buffer, tile t + gridDim.x streams into the
+// other, so HBM traffic overlaps the arithmetic inside every workgroup (one
+// per CU).  LDS: two tiles | the sweep's row table | phase factors.  Per
+// tile: wait for its DMA (all but the previous tile's kRegs stores, the
+// youngest memory operations), barrier, the tile's factors, the next tile's
+// DMA, barrier, phases, stores.
+template <int kMaxQ>
+__global__ void __launch_bounds__(kDmaThreads, 1)
+k_sweep1d(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
+          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr uint32_t kT = 1u << kDmaTileQubits;
+    c64* buf = reinterpret_cast<c64*>(smem);
+    uint64_t* ht = reinterpret_cast<uint64_t*>(buf + 2 * kT);
+    c64* xf = reinterpret_cast<c64*>(ht + (kT >> sw.c));
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+    for (uint32_t i = tid; i < (kT >> sw.c); i += kDmaThreads) ht[i] = hitab[sw.hitab + i];
+    __syncthreads();
+    const uint32_t lds0 = lds_address(buf);
+    uint64_t t = blockIdx.x;
+    tile_dma(s0, ht, sw.c, tile_base(sw, t), lds0, wave, lane);
+    uint32_t cur = 0;
+    bool first = true;
+    for (;;) {
+        // tile_store<kRegs> below issues kRegs = 16 stores per thread, so
+        // vmcnt(16) leaves only those outstanding and this tile's DMA done
+        if (first) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        else asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+        __syncthreads();
+        const uint64_t base = tile_base(sw, t);
+        const uint64_t tn = t + gridDim.x;
+        const bool more = tn < sw.n_tiles;
+        // factors first: their global loads would otherwise wait for the
+        // DMA; the explicit vmcnt(0) tells the compiler's wait analysis that
+        // no load of theirs is still in flight (a pending one whose register
+        // a later LDS read reuses would make it wait for everything)
+        tile_factors(xf, sw, ops, coef, base, tid);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0), expcnt/lgkmcnt unconstrained
+        if (more) tile_dma(s0, ht, sw.c, tile_base(sw, tn), lds0 + (cur ^ 1u) * kT * 16u, wave, lane);
+        if (sw.n_xf) __syncthreads();
+        c64* t0 = buf + cur * kT;
+        tile_phases<1, false, kMaxQ>(t0, nullptr, xf, sw, phases, ops, coef, base, tid, kDmaThreads);
+        tile_store<kRegs>(t0, 0u, s0, sw, ht, base, tid, kDmaThreads);
+        if (!more) break;
+        t = tn;
+        cur ^= 1u;
+        first = false;
+    }
+}
+
 // Two states (adjoint): two register groups per thread (one wave per SIMD),
 // dense ops of at most 2 qubits.
 __global__ void __launch_bounds__(kThreads, 1)
@@ -450,13 +557,39 @@ static uint64_t pipeline_grid(uint32_t k)
     return cache[k];
 }
 
+// Workgroups of k_sweep1d resident at once (CUs x occupancy at the largest
+// LDS it may use; 0 if it cannot run).
+static uint64_t dma_grid()
+{
+    static uint64_t cache = ~0ull;
+    if (cache != ~0ull) return cache;
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_sweep1d<kRegBits>),
+                                                     int(kDmaThreads), g_max_smem) != hipSuccess)
+        per = 0;
+    (void)hipGetLastError();
+    cache = per > 0 ? uint64_t(cus) * uint64_t(per) : 0;
+    return cache;
+}
+
 hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInfo& sw,
                         const PhaseInfo* phases, const DOp* ops, const c64* coef,
-                        const uint64_t* hitab, double* grad_partials, int pipeline)
+                        const uint64_t* hitab, double* grad_partials, int pipeline, bool prefetch)
 {
     const uint32_t nt = sweep_threads(sw.k, ns);
     const size_t smem = sweep_smem_bytes(sw, ns);
     if (smem > g_max_smem) return hipErrorInvalidConfiguration;
+    if (ns == 1 && prefetch && pipeline == 0 && sw.k == kDmaTileQubits && nt == kDmaThreads &&
+        dma_sweep_smem_bytes(sw) <= g_max_smem) {
+        const uint64_t grid = dma_grid();
+        if (grid && sw.n_tiles >= 2 * grid) {
+            hipLaunchKernelGGL(sw.max_dense <= 2 ? k_sweep1d<2> : k_sweep1d<kRegBits>, dim3(uint32_t(grid)),
+                               dim3(kDmaThreads), dma_sweep_smem_bytes(sw), st, s0, sw, phases, ops, coef, hitab);
+            return hipGetLastError();
+        }
+    }
     if (ns == 1 && (pipeline == 1 || pipeline == 2)) {
         const bool wide = pipeline == 2 && nt <= 256;
         const uint64_t resident = wide ? pipeline_grid<true>(sw.k) : pipeline_grid<false>(sw.k);
@@ -517,6 +650,12 @@ hipError_t init_sweep_kernels()
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e == hipSuccess)
             e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1p<true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1d<kRegBits>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1d<2>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e == hipSuccess)
             e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep2),

@@ -76,10 +76,12 @@ hipError_t init_sample_kernels();   // after init_sweep_kernels
 size_t sweep_smem_limit();
 size_t sweep_smem_bytes(const SweepInfo& sw, int ns);
 // pipeline != 0: one-state sweeps larger than one resident grid run
-// persistent and prefetch the next tile (k_sweep1p); 2 = 512-VGPR variant
+// persistent and prefetch the next tile (k_sweep1p); 2 = 512-VGPR variant.
+// prefetch (with pipeline 0): one-state sweeps of 12-qubit tiles over at
+// least two resident grids run k_sweep1d (double-buffered LDS DMA).
 hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInfo& sw,
                         const PhaseInfo* phases, const DOp* ops, const c64* coef,
-                        const uint64_t* hitab, double* grad_partials, int pipeline);
+                        const uint64_t* hitab, double* grad_partials, int pipeline, bool prefetch);
 hipError_t launch_standalone(hipStream_t st, c64* s, int n, const StandaloneOp& so, const c64* coef);
 hipError_t launch_fill_basis(hipStream_t st, c64* s, uint64_t dim);
 hipError_t launch_gather(hipStream_t st, const c64* s, uint64_t dim, const uint64_t* idx, uint64_t cnt, c64* out);

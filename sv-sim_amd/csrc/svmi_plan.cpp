@@ -820,6 +820,44 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
         }
         if (left == left0) throw std::logic_error("compile_phase: no item ready");
     }
+    // runs of [table] + lifts on distinct slots become one DOP_ROT (one
+    // dispatch instead of one per op): coef = the four {u, l} pairs (zero for
+    // absent slots), then the table; the emitter points grad at the table
+    std::vector<PEmit> fused;
+    fused.reserve(out.size());
+    auto is_lift = [](const PEmit& e) { return !e.s && (e.d.h.type == DOP_LIFT2 || e.d.h.type == DOP_LIFT3); };
+    for (size_t i = 0; i < out.size();) {
+        const bool tab = !out[i].s && out[i].d.h.type == DOP_RTAB;
+        size_t j = i + (tab ? 1 : 0);
+        uint32_t slots = 0;
+        while (j < out.size() && is_lift(out[j]) && !((slots >> out[j].d.h.s0) & 1u)) slots |= 1u << out[j++].d.h.s0;
+        const size_t nl = j - i - (tab ? 1 : 0);
+        if (nl == 0 || (nl == 1 && !tab)) {
+            fused.push_back(std::move(out[i]));
+            i++;
+            continue;
+        }
+        PEmit e;
+        std::memset(&e.d, 0, sizeof(e.d));
+        e.d.targets = two_state ? DOP_TO_BOTH : DOP_TO_PSI;
+        e.d.h.type = DOP_ROT;
+        e.coef.assign(4, c64{0.0, 0.0});
+        for (size_t t = i + (tab ? 1 : 0); t < j; t++) {
+            const DOpHdr& lh = out[t].d.h;
+            e.d.h.smask |= 1u << lh.s0;
+            if (lh.type == DOP_LIFT3) e.d.h.s1 |= 1u << lh.s0;
+            e.coef[lh.s0] = c64{lift_u(lh), lift_l(lh)};
+        }
+        if (tab) {
+            e.d.h.s0 = 1;
+            e.d.h.emask = out[i].d.h.emask;
+            e.coef.insert(e.coef.end(), out[i].coef.begin(), out[i].coef.end());
+        }
+        cost -= long(j - i - 1) * overhead;
+        fused.push_back(std::move(e));
+        i = j;
+    }
+    out.swap(fused);
     return cost;
 }
 
@@ -1008,6 +1046,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                     if (!em[i].s) {
                         DOp d = em[i].d;
                         d.h.coef = enc.push_coef(em[i].coef);
+                        if (d.h.type == DOP_ROT && d.h.s0) d.h.grad = d.h.coef + 4;
                         prog.ops.push_back(d);
                         i++;
                         continue;
@@ -1058,7 +1097,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     // qubits that select register-table variants become the tile id's high
     // bits: workgroups running at the same time then read the same variants
     for (size_t o = sw.op_begin; o < prog.ops.size(); o++)
-        if (prog.ops[o].h.type == DOP_RTAB) sw.exthi |= prog.ops[o].h.emask;
+        if (prog.ops[o].h.type == DOP_RTAB || (prog.ops[o].h.type == DOP_ROT && prog.ops[o].h.s0))
+            sw.exthi |= prog.ops[o].h.emask;
     sw.n_grad = n_grad_sweep;
     // lane tables of the sweep's DOP_PHASETAB ops: one contiguous copy that
     // every tile stages in LDS after its factor slots (if the area has room)
@@ -1111,6 +1151,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                 case DOP_PHASESET: b += 32L * (h.s0 + h.s1); break;
                 case DOP_PHASETAB: break;   // tables read with vector loads
                 case DOP_RTAB: b += 16L * kRegs; break;   // one variant per tile
+                case DOP_ROT: b += 64L + (h.s0 ? 16L * kRegs : 0L); break;
                 case DOP_DIAG: b += 16L << d.nq; break;
                 case DOP_CDENSE: b += 64; break;
                 case DOP_CPHASE: b += 16; break;
@@ -1311,6 +1352,9 @@ double program_fma_per_amp(const Program& prog)
                 case DOP_DENSE: f += 4.0 * (1 << __builtin_popcount(h.smask)); break;
                 case DOP_LIFT2: f += 2.0; break;
                 case DOP_LIFT3: f += 3.0; break;
+                case DOP_ROT:
+                    f += (h.s0 ? 4.0 : 0.0) + 2.0 * __builtin_popcount(h.smask) + __builtin_popcount(h.s1);
+                    break;
                 case DOP_RTAB:
                 case DOP_DIAG: f += 4.0; break;
                 case DOP_CPHASE:
@@ -1358,7 +1402,7 @@ std::string describe_program(const Program& prog)
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                     const DOp& d = prog.ops[o];
                     static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS", "CD",
-                                               "L", "L3", "T", "PT"};
+                                               "L", "L3", "T", "PT", "R"};
                     ss << " " << nm[d.h.type];
                     if (d.h.type == DOP_PHASETAB) {
                         ss << d.h.s0;
@@ -1370,6 +1414,7 @@ std::string describe_program(const Program& prog)
                     }
                     if (d.h.type == DOP_DENSE || d.h.type == DOP_GRAD_DENSE) ss << __builtin_popcount(d.h.smask);
                     if (d.h.type == DOP_RTAB && d.h.emask) ss << "e" << __builtin_popcountll(d.h.emask);
+                    if (d.h.type == DOP_ROT) ss << (d.h.s0 ? "t" : "") << __builtin_popcount(d.h.smask);
                 }
                 ss << "\n";
             }
