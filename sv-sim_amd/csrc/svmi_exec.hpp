@@ -181,7 +181,7 @@ SVMI_HD inline void apply_cphase(const DOpHdr& op, c64 (&a)[N], const c64* __res
     const uint32_t rsel = op.rsel;
 #pragma unroll
     for (int r = 0; r < N; r++)
-        if ((rsel >> (r & (kRegs - 1))) & 1u) a[r] = cmul(a[r], ph);
+        if ((rsel >> (r & (kRegs - 1))) & 1u) cmul_to(a[r], ph);
 }
 
 template <int N>
@@ -194,7 +194,7 @@ SVMI_HD inline void apply_diag(const DOp& op, c64 (&a)[N], const c64* __restrict
 #pragma unroll
     for (int r = 0; r < kRegs; r++) f[r] = tab[diag_index(op, goff | off[r], e)];
 #pragma unroll
-    for (int r = 0; r < N; r++) a[r] = cmul(a[r], f[r & (kRegs - 1)]);
+    for (int r = 0; r < N; r++) cmul_to(a[r], f[r & (kRegs - 1)]);
 }
 
 // Apply one non-gradient op to one state's register group.
@@ -227,7 +227,7 @@ SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[N], const c64* __re
     const uint32_t rsel = h.rsel;
 #pragma unroll
     for (int r = 0; r < N; r++)
-        if ((rsel >> (r & (kRegs - 1))) & 1u) a[r] = cmul(a[r], ph);
+        if ((rsel >> (r & (kRegs - 1))) & 1u) cmul_to(a[r], ph);
 }
 
 // DOP_PHASETAB: the lane's product of nibble-table entries (times the tile
@@ -250,7 +250,7 @@ SVMI_HD inline void apply_phasetab(const DOpHdr& h, c64 (&a)[N], const c64* __re
     const uint32_t rsel = h.rsel;
 #pragma unroll
     for (int r = 0; r < N; r++)
-        if ((rsel >> (r & (kRegs - 1))) & 1u) a[r] = cmul(a[r], ph);
+        if ((rsel >> (r & (kRegs - 1))) & 1u) cmul_to(a[r], ph);
 }
 
 // The tile base's bits outside the tile, packed in ascending qubit order.

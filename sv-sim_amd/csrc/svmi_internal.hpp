@@ -40,13 +40,27 @@ struct alignas(16) c64 {
 
 #if defined(__HIP_DEVICE_COMPILE__)
 #define SVMI_FMA(a, b, c) __builtin_fma((a), (b), (c))
+// x * y - z with both operations rounded (never contracted into an FMA)
+#define SVMI_MUL_SUB(x, y, z) __dsub_rn(__dmul_rn((x), (y)), (z))
 #else
 #define SVMI_FMA(a, b, c) ((a) * (b) + (c))
+#define SVMI_MUL_SUB(x, y, z) ((x) * (y) - (z))
 #endif
 
 SVMI_HD inline c64 cmul(c64 a, c64 b)
 {
     return c64{SVMI_FMA(a.re, b.re, -(a.im * b.im)), SVMI_FMA(a.re, b.im, a.im * b.re)};
+}
+// a *= b in place: the two products first, then each part's FMA is the
+// last use of that part's old value, so the result can take its input's
+// register in any order (the register groups of the sweep interpreter end
+// every op in the registers they started in; a result that cannot is
+// copied back at the join)
+SVMI_HD inline void cmul_to(c64& a, c64 b)
+{
+    const double p = a.im * b.im;
+    a.im = SVMI_FMA(a.re, b.im, a.im * b.re);
+    a.re = SVMI_MUL_SUB(a.re, b.re, p);
 }
 SVMI_HD inline c64 cadd(c64 a, c64 b) { return c64{a.re + b.re, a.im + b.im}; }
 // acc + m * v (four FMAs on the device)
@@ -440,7 +454,7 @@ template <int N = kRegs>
 SVMI_HD inline void apply_rtab(c64 (&a)[N], const c64* __restrict__ T)
 {
 #pragma unroll
-    for (int r = 0; r < N; r++) a[r] = cmul(a[r], T[r & (kRegs - 1)]);
+    for (int r = 0; r < N; r++) cmul_to(a[r], T[r & (kRegs - 1)]);
 }
 
 // Reference SRN_GATE (svsim_cpu_omp.hpp:866-879), as coded there.
