@@ -168,6 +168,34 @@ SVMI_HD inline uint32_t diag_ext_index(const DOp& op, uint64_t base)
     return idx;
 }
 
+// a[r] *= ph for the registers r whose bit is set in rsel (bit r & 15).
+// The masks of one register slot and the full mask -- the common ones --
+// run as straight code; any other mask takes one uniform branch per register.
+// (s < 0: every register)
+template <int s, int N>
+SVMI_HD inline void mul_slot(c64 (&a)[N], c64 ph)
+{
+#pragma unroll
+    for (int r = 0; r < N; r++)
+        if (s < 0 || ((r >> (s < 0 ? 0 : s)) & 1)) cmul_to(a[r], ph);
+}
+template <int N>
+SVMI_HD inline void apply_rsel(c64 (&a)[N], c64 ph, uint32_t rsel)
+{
+    switch (rsel) {
+        SVMI_CASE_ID(0xAAAAu, 400, (mul_slot<0, N>(a, ph)))
+        SVMI_CASE_ID(0xCCCCu, 401, (mul_slot<1, N>(a, ph)))
+        SVMI_CASE_ID(0xF0F0u, 402, (mul_slot<2, N>(a, ph)))
+        SVMI_CASE_ID(0xFF00u, 403, (mul_slot<3, N>(a, ph)))
+        SVMI_CASE_ID(0xFFFFu, 404, (mul_slot<-1, N>(a, ph)))
+    default:
+#pragma unroll
+        for (int r = 0; r < N; r++)
+            if ((rsel >> (r & (kRegs - 1))) & 1u) cmul_to(a[r], ph);
+        break;
+    }
+}
+
 // Controlled phase: amp *= ph where every qubit of the op is 1.  The
 // register part of the condition is the uniform mask rsel, the group part
 // one compare per thread, the part outside the tile one compare per tile.
@@ -177,11 +205,7 @@ SVMI_HD inline void apply_cphase(const DOpHdr& op, c64 (&a)[N], const c64* __res
 {
     if ((base & op.emask) != op.emask) return;
     if ((goff & op.gmask) != op.gmask) return;
-    const c64 ph = coef[op.coef];
-    const uint32_t rsel = op.rsel;
-#pragma unroll
-    for (int r = 0; r < N; r++)
-        if ((rsel >> (r & (kRegs - 1))) & 1u) cmul_to(a[r], ph);
+    apply_rsel<N>(a, coef[op.coef], op.rsel);
 }
 
 template <int N>
@@ -224,10 +248,7 @@ SVMI_HD inline void apply_phaseset(const DOpHdr& h, c64 (&a)[N], const c64* __re
         pi = on ? ti : pi;
     }
     const c64 ph{pr, pi};
-    const uint32_t rsel = h.rsel;
-#pragma unroll
-    for (int r = 0; r < N; r++)
-        if ((rsel >> (r & (kRegs - 1))) & 1u) cmul_to(a[r], ph);
+    apply_rsel<N>(a, ph, h.rsel);
 }
 
 // DOP_PHASETAB: the lane's product of nibble-table entries (times the tile
@@ -240,17 +261,19 @@ SVMI_HD inline void apply_phasetab(const DOpHdr& h, c64 (&a)[N], const c64* __re
     if (h.s1) ph = xf[h.grad];
     // two loops, so the LDS tables are read as LDS (a pointer that may be
     // either would be read with flat loads, which also wait for global ones)
+    // (at most kPhaseTabMaxLane tables: unrolled, one uniform test each)
     if (h.gmask == kLaneTabGlobal) {
         const c64* T = coef + h.coef;
-        for (uint32_t j = 0; j < h.s0; j++) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
+#pragma unroll
+        for (uint32_t j = 0; j < uint32_t(kPhaseTabMaxLane); j++)
+            if (j < h.s0) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
     } else {
         const c64* T = xf + h.gmask;
-        for (uint32_t j = 0; j < h.s0; j++) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
-    }
-    const uint32_t rsel = h.rsel;
 #pragma unroll
-    for (int r = 0; r < N; r++)
-        if ((rsel >> (r & (kRegs - 1))) & 1u) cmul_to(a[r], ph);
+        for (uint32_t j = 0; j < uint32_t(kPhaseTabMaxLane); j++)
+            if (j < h.s0) ph = cmul(ph, T[16u * j + ((grp >> (4u * j)) & 15u)]);
+    }
+    apply_rsel<N>(a, ph, h.rsel);
 }
 
 // The tile base's bits outside the tile, packed in ascending qubit order.
@@ -261,7 +284,16 @@ SVMI_HD inline c64 phasetab_factor(const DOpHdr& h, const c64* __restrict__ coef
 {
     const c64* E = coef + h.coef + 16u * h.s0;
     c64 f{1.0, 0.0};
-    for (uint32_t j = 0; j < h.smask; j++) f = cmul(f, E[16u * j + uint32_t((x >> (4u * j)) & 15u)]);
+    if (h.smask == 0) return f;
+    // every table's entry is loaded before the first product, so the loads
+    // overlap; tables past h.smask read entry 0 of the first and are unused
+    c64 e[kPhaseTabMaxExt];
+#pragma unroll
+    for (uint32_t j = 0; j < uint32_t(kPhaseTabMaxExt); j++)
+        e[j] = E[j < h.smask ? 16u * j + uint32_t((x >> (4u * j)) & 15u) : 0u];
+#pragma unroll
+    for (uint32_t j = 0; j < uint32_t(kPhaseTabMaxExt); j++)
+        if (j < h.smask) f = cmul(f, e[j]);
     return f;
 }
 
