@@ -321,6 +321,10 @@ SVMI_HD inline void apply_dense(c64 (&a)[N], const c64* __restrict__ M)
 
 // Re sum_groups conj(l) . (M p) over the slots of S (state untouched);
 // pl = phi's H amplitudes then lambda's (S within the low log2(H) slots).
+// Written as sum_ij (Re M_ij R_ij - Im M_ij I_ij) with R_ij + i I_ij =
+// sum_o conj(l_i) p_j: each component of M is a uniform value, and a zero
+// one (the usual case: U^dagger dU of RY is real antisymmetric, of a phase
+// gate a single imaginary entry) skips its two FMAs per pair.
 template <int S, int H>
 SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__ M)
 {
@@ -328,17 +332,33 @@ SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__
     constexpr int D = 1 << q;
     double sum = 0.0;
 #pragma unroll
-    for (int o = 0; o < (H >> q); o++) {
+    for (int i = 0; i < D; i++)
 #pragma unroll
-        for (int i = 0; i < D; i++) {
-            c64 acc = c64{0.0, 0.0};
+        for (int j = 0; j < D; j++) {
+            const c64 m = M[i * D + j];
+            if (m.re != 0.0) {
+                double r = 0.0;
 #pragma unroll
-            for (int j = 0; j < D; j++)
-                acc = cmac(acc, M[i * D + j], pl[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)]);
-            const c64 lv = pl[H + (SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i))];
-            sum += lv.re * acc.re + lv.im * acc.im;
+                for (int o = 0; o < (H >> q); o++) {
+                    const c64 lv = pl[H + (SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i))];
+                    const c64 pv = pl[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)];
+                    r = SVMI_FMA(lv.re, pv.re, r);
+                    r = SVMI_FMA(lv.im, pv.im, r);
+                }
+                sum = SVMI_FMA(m.re, r, sum);
+            }
+            if (m.im != 0.0) {
+                double t = 0.0;
+#pragma unroll
+                for (int o = 0; o < (H >> q); o++) {
+                    const c64 lv = pl[H + (SlotMask<S>::spread_out(o) | SlotMask<S>::spread(i))];
+                    const c64 pv = pl[SlotMask<S>::spread_out(o) | SlotMask<S>::spread(j)];
+                    t = SVMI_FMA(lv.re, pv.im, t);
+                    t = SVMI_FMA(-lv.im, pv.re, t);
+                }
+                sum = SVMI_FMA(-m.im, t, sum);
+            }
         }
-    }
     return sum;
 }
 

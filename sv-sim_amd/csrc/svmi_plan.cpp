@@ -289,6 +289,19 @@ struct Encoder {
         return off;
     }
 
+    // A gradient op's K = U^dagger dU: components within 1e-15 of zero are
+    // rounding residue of exact zeros (K of RY is real antisymmetric, of a
+    // phase gate one imaginary entry) and are stored as zero, which the
+    // device reduction skips.
+    uint32_t push_grad_coef(std::vector<c64> v)
+    {
+        for (c64& x : v) {
+            if (std::fabs(x.re) < 1e-15) x.re = 0.0;
+            if (std::fabs(x.im) < 1e-15) x.im = 0.0;
+        }
+        return push_coef(v);
+    }
+
     // Encode op for a phase: tloc[p] = tile-local bit of physical qubit p (or -1),
     // slot_of[l] = register slot of local bit l (or -1).
     void encode(const SOp& s, const int* tloc, const int* slot_of, uint32_t grad_slot)
@@ -301,6 +314,16 @@ struct Encoder {
         if (op.kind == LOp::SRN) {
             d.h.type = DOP_SRN;
             d.h.s0 = uint32_t(slot_of[tloc[op.q[0]]]);
+            prog.ops.push_back(d);
+            return;
+        }
+        if (op.kind == LOp::DIAG && s.grad && op.k == 1 && op.ctrl == 0 && tloc[op.q[0]] >= 0 &&
+            slot_of[tloc[op.q[0]]] >= 0) {
+            // the gradient of a one-qubit diagonal on a register slot: a 2x2
+            // K = diag(m0, m1) for the sparse dense-gradient reduction
+            d.h.type = DOP_GRAD_DENSE;
+            d.h.smask = 1u << slot_of[tloc[op.q[0]]];
+            d.h.coef = push_grad_coef(std::vector<c64>{op.m[0], c64{0.0, 0.0}, c64{0.0, 0.0}, op.m[1]});
             prog.ops.push_back(d);
             return;
         }
@@ -326,7 +349,7 @@ struct Encoder {
                 d.h.coef = push_coef(std::vector<c64>{op.m[D - 1]});
             } else {
                 d.h.type = s.grad ? DOP_GRAD_DIAG : DOP_DIAG;
-                d.h.coef = push_coef(op.m);
+                d.h.coef = s.grad ? push_grad_coef(op.m) : push_coef(op.m);
                 if (op.dyn >= 0) prog.measures.at(size_t(op.dyn)).coef = d.h.coef;
             }
             prog.ops.push_back(d);
@@ -399,7 +422,7 @@ struct Encoder {
         d.h.type = s.grad ? DOP_GRAD_DENSE : DOP_DENSE;
         d.h.smask = 0;
         for (int i = 0; i < op.k; i++) d.h.smask |= 1u << slots[i];
-        d.h.coef = push_coef(M);
+        d.h.coef = s.grad ? push_grad_coef(M) : push_coef(M);
         prog.ops.push_back(d);
     }
 };
@@ -583,7 +606,15 @@ struct RunMerger {
 long kept_cost(const SOp& s, long overhead)
 {
     const LOp& op = *s.op;
-    if (s.grad) return (op.kind == LOp::DIAG ? 6 : (8L << op.k)) + overhead;
+    if (s.grad) {
+        if (op.kind == LOp::DIAG && op.k > 1) return 6 + overhead;
+        // sparse reduction: two FMAs per pair and nonzero component of K
+        const size_t D = size_t(1) << op.k;
+        long nz = 0;
+        for (size_t i = 0; i < (op.kind == LOp::DIAG ? D : D * D); i++)
+            nz += (std::fabs(op.m[i].re) >= 1e-15) + (std::fabs(op.m[i].im) >= 1e-15);
+        return std::max(1L, 2 * nz / long(D)) + overhead;
+    }
     if (op.kind == LOp::DIAG || op.kind == LOp::SRN) return 4 + overhead;
     if (is_register_perm(op)) return 1 + overhead;
     return (4L << op.k) + (op.k == 1 ? 2 : 0) + overhead;   // 2x2: plus the copies at the join
@@ -1362,7 +1393,14 @@ double program_fma_per_amp(const Program& prog)
                 case DOP_PHASETAB: f += 4.0 * sel; break;
                 case DOP_CDENSE: f += 8.0 * sel; break;
                 case DOP_SRN: f += 2.0; break;
-                case DOP_GRAD_DENSE: f += 8.0 * (1 << __builtin_popcount(h.smask)); break;
+                case DOP_GRAD_DENSE: {
+                    const int D = 1 << __builtin_popcount(h.smask);
+                    int nz = 0;
+                    for (int i = 0; i < D * D; i++)
+                        nz += (prog.coef[h.coef + i].re != 0.0) + (prog.coef[h.coef + i].im != 0.0);
+                    f += 2.0 * nz / D;
+                    break;
+                }
                 case DOP_GRAD_DIAG: f += 6.0; break;
                 default: break;
                 }
