@@ -42,12 +42,12 @@ SVMI_HD inline void dispatch_dense(int smask, c64 (&a)[N], const c64* __restrict
 }
 
 template <int kMaxQ, int H>
-SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&pl)[2 * H], const c64* __restrict__ M)
+SVMI_HD inline double dispatch_grad_dense(int smask, const c64 (&pl)[2 * H], const c64* __restrict__ M, uint32_t nz)
 {
     double sum = 0.0;
     switch (smask) {
 #define SVMI_GRAD_CASE(S) \
-    SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ && (S & ~(H - 1)) == 0) sum = (grad_dense<S, H>(pl, M)))
+    SVMI_CASE(S, if constexpr (cpopc(S) <= kMaxQ && (S & ~(H - 1)) == 0) sum = (grad_dense<S, H>(pl, M, nz)))
         SVMI_GRAD_CASE(1) SVMI_GRAD_CASE(2) SVMI_GRAD_CASE(3) SVMI_GRAD_CASE(4)
         SVMI_GRAD_CASE(5) SVMI_GRAD_CASE(6) SVMI_GRAD_CASE(7) SVMI_GRAD_CASE(8)
         SVMI_GRAD_CASE(9) SVMI_GRAD_CASE(10) SVMI_GRAD_CASE(11) SVMI_GRAD_CASE(12)
@@ -404,7 +404,7 @@ SVMI_HD inline double exec_grad_op(const DOpHdr& h, const DOp* __restrict__ op, 
                                    const c64* __restrict__ coef, uint32_t goff, const uint32_t (&off)[kRegs],
                                    uint64_t base)
 {
-    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense<kMaxQ, H>(h.smask, pl, coef + h.coef);
+    if (h.type == DOP_GRAD_DENSE) return dispatch_grad_dense<kMaxQ, H>(h.smask, pl, coef + h.coef, h.s0);
     // DOP_GRAD_DIAG
     const uint32_t e = diag_ext_index(*op, base);
     const c64* tab = coef + h.coef;

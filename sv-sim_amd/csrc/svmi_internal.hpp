@@ -105,7 +105,8 @@ enum DOpType : uint16_t {
     DOP_SRN = 5,     // reference SRN quirk on slot s0 (real-linear only)
     DOP_CPHASE = 6,  // amp *= coef[0] where all bits of the op are set
     DOP_DIAG = 7,    // amp *= coef[idx], idx from the op's nq qubits
-    DOP_GRAD_DENSE = 8,  // adjoint: acc[grad] += Re sum conj(lam) (coef-matrix phi) on slots
+    DOP_GRAD_DENSE = 8,  // adjoint: acc[grad] += Re sum conj(lam) (coef-matrix phi) on slots;
+                         // s0 = the matrix's nonzero components (grad_dense)
     DOP_GRAD_DIAG = 9,   // adjoint: acc[grad] += Re sum conj(lam) coef[idx] phi
     DOP_PHASESET = 10,   // product of controlled phases, applied to registers rsel
     DOP_CDENSE = 11,     // 2x2 coef on register slot s0, on the register pairs in rsel
@@ -322,11 +323,13 @@ SVMI_HD inline void apply_dense(c64 (&a)[N], const c64* __restrict__ M)
 // Re sum_groups conj(l) . (M p) over the slots of S (state untouched);
 // pl = phi's H amplitudes then lambda's (S within the low log2(H) slots).
 // Written as sum_ij (Re M_ij R_ij - Im M_ij I_ij) with R_ij + i I_ij =
-// sum_o conj(l_i) p_j: each component of M is a uniform value, and a zero
-// one (the usual case: U^dagger dU of RY is real antisymmetric, of a phase
-// gate a single imaginary entry) skips its two FMAs per pair.
+// sum_o conj(l_i) p_j: a zero component of M (the usual case: U^dagger dU
+// of RY is real antisymmetric, of a phase gate a single imaginary entry)
+// skips its two FMAs per pair.  nz (from the op header, so no branch waits
+// for M's loads): bit 2e is set if Re M_e != 0, bit 2e + 1 if Im M_e != 0,
+// e = i * D + j.
 template <int S, int H>
-SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__ M)
+SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__ M, uint32_t nz)
 {
     constexpr int q = SlotMask<S>::q;
     constexpr int D = 1 << q;
@@ -335,8 +338,9 @@ SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__
     for (int i = 0; i < D; i++)
 #pragma unroll
         for (int j = 0; j < D; j++) {
-            const c64 m = M[i * D + j];
-            if (m.re != 0.0) {
+            const int e = i * D + j;
+            if ((nz >> (2 * e)) & 1u) {
+                const double mre = M[e].re;
                 double r = 0.0;
 #pragma unroll
                 for (int o = 0; o < (H >> q); o++) {
@@ -345,9 +349,10 @@ SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__
                     r = SVMI_FMA(lv.re, pv.re, r);
                     r = SVMI_FMA(lv.im, pv.im, r);
                 }
-                sum = SVMI_FMA(m.re, r, sum);
+                sum = SVMI_FMA(mre, r, sum);
             }
-            if (m.im != 0.0) {
+            if ((nz >> (2 * e + 1)) & 1u) {
+                const double mim = M[e].im;
                 double t = 0.0;
 #pragma unroll
                 for (int o = 0; o < (H >> q); o++) {
@@ -356,7 +361,7 @@ SVMI_HD inline double grad_dense(const c64 (&pl)[2 * H], const c64* __restrict__
                     t = SVMI_FMA(lv.re, pv.im, t);
                     t = SVMI_FMA(-lv.im, pv.re, t);
                 }
-                sum = SVMI_FMA(-m.im, t, sum);
+                sum = SVMI_FMA(-mim, t, sum);
             }
         }
     return sum;

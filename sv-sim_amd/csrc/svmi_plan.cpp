@@ -302,6 +302,19 @@ struct Encoder {
         return push_coef(v);
     }
 
+    // DOP_GRAD_DENSE's component mask: bit 2e for Re, 2e + 1 for Im of entry e
+    // (gradient ops act on at most two register slots: n <= 16 entries)
+    uint32_t nonzero_mask(uint32_t off, uint32_t n) const
+    {
+        if (n > 16) throw std::logic_error("gradient op wider than two register slots");
+        uint32_t nz = 0;
+        for (uint32_t e = 0; e < n; e++) {
+            if (prog.coef[off + e].re != 0.0) nz |= 1u << (2 * e);
+            if (prog.coef[off + e].im != 0.0) nz |= 1u << (2 * e + 1);
+        }
+        return nz;
+    }
+
     // Encode op for a phase: tloc[p] = tile-local bit of physical qubit p (or -1),
     // slot_of[l] = register slot of local bit l (or -1).
     void encode(const SOp& s, const int* tloc, const int* slot_of, uint32_t grad_slot)
@@ -324,6 +337,7 @@ struct Encoder {
             d.h.type = DOP_GRAD_DENSE;
             d.h.smask = 1u << slot_of[tloc[op.q[0]]];
             d.h.coef = push_grad_coef(std::vector<c64>{op.m[0], c64{0.0, 0.0}, c64{0.0, 0.0}, op.m[1]});
+            d.h.s0 = nonzero_mask(d.h.coef, 4);
             prog.ops.push_back(d);
             return;
         }
@@ -423,6 +437,7 @@ struct Encoder {
         d.h.smask = 0;
         for (int i = 0; i < op.k; i++) d.h.smask |= 1u << slots[i];
         d.h.coef = s.grad ? push_grad_coef(M) : push_coef(M);
+        if (s.grad) d.h.s0 = nonzero_mask(d.h.coef, uint32_t(D * D));
         prog.ops.push_back(d);
     }
 };
