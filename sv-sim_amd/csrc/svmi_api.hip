@@ -393,6 +393,12 @@ svmi_status verify_batch(const BatchPlan& bp)
                     if (state_bit ? ph.r[r] != sw.k : ph.r[r] >= sw.k)
                         return fail(SVMI_ERR_STATE, "batch plan: bad register bit");
                 }
+                // the gather map moves amplitudes only among the register bits
+                uint32_t regmask = 0;
+                for (int r = 0; r < kRegBits; r++) regmask |= 1u << ph.r[r];
+                bool map_ok = (ph.gxor & ~regmask) == 0;
+                for (int r = 0; r < kRegBits; r++) map_ok = map_ok && ph.gcol[r] && (ph.gcol[r] & ~regmask) == 0;
+                if (!map_ok) return fail(SVMI_ERR_STATE, "batch plan: bad gather map");
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++)
                     if (bp.ops[o].h.coef > bp.coef.size()) return fail(SVMI_ERR_STATE, "batch plan: bad coef offset");
             }
@@ -1647,12 +1653,13 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
     }
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
         const PhaseInfo& ph = phases[p];
-        uint32_t off[kRegs];
+        uint32_t off[kRegs], goffs[kRegs];
         phase_offsets(ph, off);
+        phase_gather_offsets(ph, goffs);
         for (uint32_t g = 0; g < ngroups; g++) {
             const uint32_t goff = group_offset(ph, g);
             c64 a[kRegs];
-            for (int r = 0; r < kRegs; r++) a[r] = t[goff | off[r]];
+            for (int r = 0; r < kRegs; r++) a[r] = t[goff | goffs[r]];
             for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                 const DOp& op = ops[o];
                 if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG)

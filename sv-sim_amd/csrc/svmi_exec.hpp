@@ -430,6 +430,20 @@ SVMI_HD inline void phase_offsets(const PhaseInfo& ph, uint32_t (&off)[kRegs])
     }
 }
 
+// Gather offsets of a phase (PhaseInfo::gcol / gxor): the register contents
+// after its folded leading permutations.
+SVMI_HD inline void phase_gather_offsets(const PhaseInfo& ph, uint32_t (&off)[kRegs])
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) {
+        uint32_t o = ph.gxor;
+#pragma unroll
+        for (int s = 0; s < kRegBits; s++)
+            if ((r >> s) & 1) o ^= ph.gcol[s];
+        off[r] = o;
+    }
+}
+
 // Tile-local base index of group g: insert zero bits at the 4 register bits.
 SVMI_HD inline uint32_t group_offset(const PhaseInfo& ph, uint32_t g)
 {

@@ -217,7 +217,16 @@ static_assert(sizeof(DOpHdr) == 40 && sizeof(DOp) == 88, "DOp layout");
 struct PhaseInfo {
     uint32_t r[4];       // register bits (tile-local positions, ascending)
     uint32_t op_begin, op_end;
+    // gather map: register rho reads tile-local offset gxor ^ (XOR of gcol[s]
+    // over the set bits s of rho) (within the group).  The phase's leading
+    // register permutations (X, CX, SWAP on register slots) are folded into it
+    // instead of being run; without them gcol[s] = 1 << r[s] and gxor = 0.
+    // Everything after the gather uses the plain map (1 << r[s]).
+    uint32_t gcol[4];
+    uint32_t gxor;
+    uint32_t pad;
 };
+static_assert(sizeof(PhaseInfo) == 48, "PhaseInfo layout");
 
 struct SweepInfo {
     uint32_t phase_begin, phase_end;

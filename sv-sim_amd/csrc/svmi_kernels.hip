@@ -103,15 +103,16 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
         if (active) {
             const PhaseInfo ph = phases[p];
-            uint32_t off[kRegs];
+            uint32_t off[kRegs], goffs[kRegs];
             phase_offsets(ph, off);
+            phase_gather_offsets(ph, goffs);
             const uint32_t goff = group_offset(ph, tid);
             // NS = 2: the phase's last register bit is k, so a[0..7] are
             // phi's amplitudes and a[8..15] lambda's (t1 = t0 + 2^k) and every
             // op, dispatched once, acts on both
             c64 a[kRegs];
 #pragma unroll
-            for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(goff | off[r])];
+            for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(goff | goffs[r])];
             // op headers are fetched one op ahead so the scalar-load latency
             // of the next op overlaps the arithmetic of the current one (every
             // op array ends with a NOP, so the read past a phase's last op

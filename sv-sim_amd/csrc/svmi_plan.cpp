@@ -907,6 +907,25 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
     return cost;
 }
 
+// The phase's leading register permutations become its gather map: after
+// them register rho holds what the gather would have put in register P(rho),
+// so the map g becomes g o P (X on slot s: gxor ^= gcol[s]; CX control c,
+// target t: gcol[c] ^= gcol[t]; SWAP: exchange two columns) and the ops are
+// skipped.
+void fold_leading_permutations(const Program& prog, PhaseInfo& pi)
+{
+    for (int s = 0; s < kRegBits; s++) pi.gcol[s] = 1u << pi.r[s];
+    pi.gxor = 0;
+    while (pi.op_begin < pi.op_end) {
+        const DOpHdr& h = prog.ops[pi.op_begin].h;
+        if (h.type == DOP_X) pi.gxor ^= pi.gcol[h.s0];
+        else if (h.type == DOP_CX) pi.gcol[h.s0] ^= pi.gcol[h.s1];
+        else if (h.type == DOP_SWAP) std::swap(pi.gcol[h.s0], pi.gcol[h.s1]);
+        else break;
+        pi.op_begin++;
+    }
+}
+
 // Emit a standalone full-state op step.
 void emit_standalone(Program& prog, const SOp& s)
 {
@@ -1136,6 +1155,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             enc.encode(*s, tloc, slot_of, gslot);
         }
         pi.op_end = uint32_t(prog.ops.size());
+        fold_leading_permutations(prog, pi);
         prog.phases.push_back(pi);
         remaining.swap(rest);
     }
