@@ -396,7 +396,7 @@ svmi_status verify_batch(const BatchPlan& bp)
                 // the gather map moves amplitudes only among the register bits
                 uint32_t regmask = 0;
                 for (int r = 0; r < kRegBits; r++) regmask |= 1u << ph.r[r];
-                bool map_ok = (ph.gxor & ~regmask) == 0;
+                bool map_ok = (ph.gxor & ~regmask) == 0 && (ph.sxor & ~regmask) == 0;
                 for (int r = 0; r < kRegBits; r++) map_ok = map_ok && ph.gcol[r] && (ph.gcol[r] & ~regmask) == 0;
                 if (!map_ok) return fail(SVMI_ERR_STATE, "batch plan: bad gather map");
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++)
@@ -1667,7 +1667,7 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
                 else
                     exec_state_op(op.h, &op, a, coef, goff, g, off, base, xf.data());
             }
-            for (int r = 0; r < kRegs; r++) t[goff | off[r]] = a[r];
+            for (int r = 0; r < kRegs; r++) t[(goff | off[r]) ^ ph.sxor] = a[r];
         }
     }
     for (uint32_t l = 0; l < tsize; l++) {

@@ -224,7 +224,9 @@ struct PhaseInfo {
     // Everything after the gather uses the plain map (1 << r[s]).
     uint32_t gcol[4];
     uint32_t gxor;
-    uint32_t pad;
+    // scatter map: register rho is written to tile-local offset off[rho] ^ sxor
+    // (the quarter turns a phase keeps in its register coordinates)
+    uint32_t sxor;
 };
 static_assert(sizeof(PhaseInfo) == 48, "PhaseInfo layout");
 

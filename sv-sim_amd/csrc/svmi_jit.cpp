@@ -178,13 +178,14 @@ void emit_sweep(std::ostringstream& o, const Program& prog, uint32_t i)
         const PhaseInfo& ph = prog.phases[p];
         o << "    {\n        constexpr PhaseInfo P = {{" << ph.r[0] << "u, " << ph.r[1] << "u, " << ph.r[2] << "u, "
           << ph.r[3] << "u}, 0u, 0u, {" << ph.gcol[0] << "u, " << ph.gcol[1] << "u, " << ph.gcol[2] << "u, "
-          << ph.gcol[3] << "u}, " << ph.gxor << "u, 0u};\n"
+          << ph.gcol[3] << "u}, " << ph.gxor << "u, " << ph.sxor << "u};\n"
           << "        uint32_t off[kRegs], goffs[kRegs];\n        phase_offsets(P, off);\n"
           << "        phase_gather_offsets(P, goffs);\n"
           << "        const uint32_t goff = group_offset(P, tid);\n        c64 a[kRegs];\n"
           << "#pragma unroll\n        for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(goff | goffs[r])];\n";
         for (uint32_t op = ph.op_begin; op < ph.op_end; op++) emit_state_op(o, prog.ops[op], ph);
-        o << "#pragma unroll\n        for (int r = 0; r < kRegs; r++) t0[lds_slot(goff | off[r])] = a[r];\n"
+        o << "#pragma unroll\n        for (int r = 0; r < kRegs; r++) t0[lds_slot((goff | off[r]) ^ " << ph.sxor
+          << "u)] = a[r];\n"
           << "    }\n    __syncthreads();\n";
     }
     o << "#pragma unroll\n    for (int j = 0; j < " << kRegs << "; j++) {\n"

@@ -141,7 +141,7 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
                 exec_state_op<kMaxQ>(h, op, a, coef, goff, tid, off, base, xf);
             }
 #pragma unroll
-            for (int r = 0; r < kRegs; r++) t0[lds_slot(goff | off[r])] = a[r];
+            for (int r = 0; r < kRegs; r++) t0[lds_slot((goff | off[r]) ^ ph.sxor)] = a[r];
         }
         __syncthreads();
     }

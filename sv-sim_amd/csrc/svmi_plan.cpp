@@ -685,7 +685,114 @@ struct PEmit {
     const SOp* s = nullptr;
     DOp d;
     std::vector<c64> coef;
+    uint32_t flip = 0;   // kept controlled phase: register slots held as quarter turns there
 };
+
+// A register mask rsel in quarter-turn coordinates f: register r holds the
+// physical register r ^ f.
+uint32_t flip_rsel(uint32_t rsel, uint32_t f)
+{
+    uint32_t out = 0;
+    for (uint32_t r = 0; r < uint32_t(kRegs); r++)
+        if ((rsel >> (r ^ f)) & 1u) out |= 1u << r;
+    return out;
+}
+
+// Quarter turns kept in the register coordinates (one-state phases).  A
+// rotation R(phi) with cos < sin runs as three shears (DOP_LIFT3).  Written
+// R(phi) = Q R(phi - pi/2) with Q = R(pi/2) it needs two (DOP_LIFT2 of
+// phi - pi/2, whose cosine sin(phi) is the larger) and Q, which is not run:
+// from then on the slot's registers hold Q^-1 of the physical pair
+// (x0 = p1, x1 = -p0; f = the slots so held), and the ops after it are
+// rewritten in those coordinates -- a table entry for register r is the
+// physical entry of r ^ f; a DOP_LIFT2 (diag(1/c, c) R, its diag(c, 1/c)
+// in the next table) becomes the same lift plus diag(c^2, 1/c^2); a
+// DOP_LIFT3 (pure rotation) stays; a second quarter turn on a slot leaves -1
+// on its pair.  Such diagonal factors wait for the next table; the phase's
+// last table also takes the signs (-1)^popcount(r & f), and the scatter
+// writes register r to the physical index r ^ f (PhaseInfo::sxor).  Kept
+// controlled phases (DOP_CPHASE / phase products: diagonal, a register mask
+// rsel) apply where register r ^ f meets their condition (PEmit::flip; the
+// emitter rewrites rsel).  A phase is converted only if every op from its
+// first three-shear rotation on is a table, a rotation or such a phase, and
+// the last one is a table.  Returns f; the number of conversions (one FMA
+// per amplitude each) goes to *saved.
+uint32_t quarter_turns(std::vector<PEmit>& out, long* saved)
+{
+    *saved = 0;
+    size_t i0 = 0;
+    while (i0 < out.size() && (out[i0].s || out[i0].d.h.type != DOP_LIFT3)) i0++;
+    if (i0 == out.size()) return 0;
+    for (size_t j = i0; j < out.size(); j++) {
+        if (out[j].s) {
+            if (!is_cphase(*out[j].s) || out[j].s->grad || out[j].s->op->ctrl) return 0;
+            continue;
+        }
+        const uint32_t t = out[j].d.h.type;
+        if (t != DOP_LIFT2 && t != DOP_LIFT3 && t != DOP_RTAB) return 0;
+    }
+    if (out.back().s || out.back().d.h.type != DOP_RTAB) return 0;
+    // dry run on copies: a lift on a slot with a factor still waiting aborts
+    std::vector<PEmit> res(out.begin(), out.end());
+    uint32_t f = 0, pend_mask = 0;
+    c64 pend[kRegBits][2];
+    for (int q = 0; q < kRegBits; q++) pend[q][0] = pend[q][1] = c64{1.0, 0.0};
+    auto wait = [&](int q, double d0, double d1) {
+        pend[q][0] = c64{pend[q][0].re * d0, pend[q][0].im * d0};
+        pend[q][1] = c64{pend[q][1].re * d1, pend[q][1].im * d1};
+        pend_mask |= 1u << q;
+    };
+    long conv = 0;
+    for (size_t j = i0; j < res.size(); j++) {
+        if (res[j].s) {   // kept controlled phase: diagonal, commutes with the waiting factors
+            res[j].flip = f;
+            continue;
+        }
+        DOpHdr& h = res[j].d.h;
+        if (h.type == DOP_RTAB) {
+            std::vector<c64>& T = res[j].coef;
+            const std::vector<c64> T0 = T;
+            for (size_t v = 0; v < T.size() / kRegs; v++)
+                for (uint32_t r = 0; r < uint32_t(kRegs); r++) {
+                    c64 e = T0[v * kRegs + (r ^ f)];
+                    for (int q = 0; q < kRegBits; q++)
+                        if ((pend_mask >> q) & 1u) e = cmul(e, pend[q][(r >> q) & 1u]);
+                    T[v * kRegs + r] = e;
+                }
+            for (int q = 0; q < kRegBits; q++) pend[q][0] = pend[q][1] = c64{1.0, 0.0};
+            pend_mask = 0;
+            continue;
+        }
+        const int q = int(h.s0);
+        if ((pend_mask >> q) & 1u) return 0;
+        const double u = lift_u(h), l = lift_l(h);
+        if (h.type == DOP_LIFT2) {
+            if ((f >> q) & 1u) {
+                const double c2 = u == 0.0 ? 1.0 : -l / u;   // u = -s/c, l = s c
+                wait(q, c2, 1.0 / c2);
+            }
+            continue;
+        }
+        // DOP_LIFT3: u = (c - 1)/s, l = s; as two shears of phi - pi/2
+        const double sn = l, cs = 1.0 + u * l;
+        const double u2 = cs / sn, l2 = -cs * sn;
+        h.type = DOP_LIFT2;
+        lift_set(h, u2, l2);
+        res[j].coef.assign(1, c64{u2, l2});
+        wait(q, sn, 1.0 / sn);
+        if ((f >> q) & 1u) wait(q, -1.0, -1.0);
+        f ^= 1u << q;
+        conv++;
+    }
+    // the scatter signs on the last table (pend_mask is 0 after it)
+    std::vector<c64>& T = res.back().coef;
+    for (size_t v = 0; v < T.size() / kRegs; v++)
+        for (uint32_t r = 0; r < uint32_t(kRegs); r++)
+            if (__builtin_popcount(r & f) & 1) T[v * kRegs + r] = c64{-T[v * kRegs + r].re, -T[v * kRegs + r].im};
+    out.swap(res);
+    *saved = conv;
+    return f;
+}
 
 // Compile the phase's op list; returns the cost of the result.  two_state:
 // an adjoint phase (every state op acts on phi and lambda, gradient ops read
@@ -696,8 +803,9 @@ struct PEmit {
 // (diagonal K) commutes with every diagonal item and (dense K) with every
 // item on other qubits.
 long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const int* slot_of, long overhead,
-                   bool two_state, std::vector<PEmit>& out)
+                   bool two_state, std::vector<PEmit>& out, uint32_t* flip)
 {
+    *flip = 0;
     out.clear();
     std::vector<PItem> items;
     items.reserve(ops.size() * 3);
@@ -865,6 +973,11 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
             }
         }
         if (left == left0) throw std::logic_error("compile_phase: no item ready");
+    }
+    if (!two_state) {
+        long saved = 0;
+        *flip = quarter_turns(out, &saved);
+        cost -= saved;
     }
     // runs of [table] + lifts on distinct slots become one DOP_ROT (one
     // dispatch instead of one per op): coef = the four {u, l} pairs (zero for
@@ -1103,7 +1216,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             for (const SOp* s : ph) rm.add(s);
             rm.flush();
             std::vector<PEmit> em;
-            const long c_rot = compile_phase(merged_rot, tloc, slot_of, opt.op_overhead, two, em);
+            uint32_t flip = 0;
+            const long c_rot = compile_phase(merged_rot, tloc, slot_of, opt.op_overhead, two, em, &flip);
             long c_kept = 0;
             for (const SOp* s : merged) c_kept += kept_cost(*s, opt.op_overhead);
             if (c_rot < c_kept) {
@@ -1116,24 +1230,35 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                         i++;
                         continue;
                     }
+                    const size_t first_op = prog.ops.size();
+                    const uint32_t fl = em[i].flip;
                     size_t j = i;
-                    while (j < em.size() && em[j].s && is_cphase(*em[j].s) && !em[j].s->grad) j++;
+                    while (j < em.size() && em[j].s && is_cphase(*em[j].s) && !em[j].s->grad && em[j].flip == fl) j++;
                     if (j - i >= 2) {
                         std::vector<const SOp*> run;
                         for (size_t t = i; t < j; t++) run.push_back(em[t].s);
                         enc.encode_phasesets(run, 0, run.size(), tloc, slot_of);
                         i = j;
-                        continue;
+                    } else {
+                        const SOp* s = em[i++].s;
+                        uint32_t gslot = 0;
+                        if (s->grad) {
+                            gslot = n_grad_sweep++;
+                            prog.grad_param.push_back(s->param);
+                        }
+                        enc.encode(*s, tloc, slot_of, gslot);
                     }
-                    const SOp* s = em[i++].s;
-                    uint32_t gslot = 0;
-                    if (s->grad) {
-                        gslot = n_grad_sweep++;
-                        prog.grad_param.push_back(s->param);
-                    }
-                    enc.encode(*s, tloc, slot_of, gslot);
+                    if (fl)   // controlled phases under quarter turns: their register masks
+                        for (size_t o = first_op; o < prog.ops.size(); o++) {
+                            DOpHdr& h = prog.ops[o].h;
+                            if (h.type != DOP_CPHASE && h.type != DOP_PHASESET && h.type != DOP_PHASETAB)
+                                throw std::logic_error("quarter turns: a kept op without a register mask");
+                            h.rsel = flip_rsel(h.rsel, fl);
+                        }
                 }
                 merged.clear();
+                for (int q = 0; q < kRegBits; q++)
+                    if ((flip >> q) & 1u) pi.sxor |= 1u << pi.r[q];
             }
         }
         for (size_t i = 0; i < merged.size();) {
