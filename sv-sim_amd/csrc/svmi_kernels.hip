@@ -461,24 +461,44 @@ k_batch_pauli(BatchArgs a, uint32_t s0, uint32_t s1, uint64_t task0)
     const uint64_t dim = 1ull << c.n;
     const uint64_t y0 = task * kPauliChunk;
     const uint64_t y1 = y0 + kPauliChunk < dim ? y0 + kPauliChunk : dim;
-    double acc = 0.0;
-    for (uint64_t y = y0 + threadIdx.x; y < y1; y += blockDim.x) {
-        c64 l{0.0, 0.0};
-        for (uint32_t g = c.group_begin; g < c.group_end; g++) {
-            const PauliGroup G = a.groups[g];
-            const uint64_t x = y ^ G.xmask;
-            double fr = 0.0, fi = 0.0;
-            for (uint32_t e = G.begin; e < G.end; e++) {
-                const PauliTerm T = a.terms[e];
+    // the thread's kPauliPerThread amplitudes y0 + tid + 256 j in registers,
+    // groups and terms outermost: each term is read once per thread, not
+    // once per amplitude (same sums in the same order as one amplitude at a
+    // time)
+    constexpr int M = kPauliChunk / kThreads;
+    c64 l[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) l[j] = c64{0.0, 0.0};
+    for (uint32_t g = c.group_begin; g < c.group_end; g++) {
+        const PauliGroup G = a.groups[g];
+        double fr[M], fi[M];
+#pragma unroll
+        for (int j = 0; j < M; j++) fr[j] = fi[j] = 0.0;
+        for (uint32_t e = G.begin; e < G.end; e++) {
+            const PauliTerm T = a.terms[e];
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                const uint64_t x = (y0 + threadIdx.x + uint64_t(kThreads) * j) ^ G.xmask;
                 const bool neg = __popcll(x & T.zmask) & 1;
-                fr += neg ? -T.cre : T.cre;
-                fi += neg ? -T.cim : T.cim;
+                fr[j] += neg ? -T.cre : T.cre;
+                fi[j] += neg ? -T.cim : T.cim;
             }
-            l = cmac(l, c64{fr, fi}, c.phi[x]);
         }
-        c.lam[y] = l;
-        const c64 p = c.phi[y];
-        acc += p.re * l.re + p.im * l.im;
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            const uint64_t y = y0 + threadIdx.x + uint64_t(kThreads) * j;
+            if (y < y1) l[j] = cmac(l[j], c64{fr[j], fi[j]}, c.phi[y ^ G.xmask]);
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        const uint64_t y = y0 + threadIdx.x + uint64_t(kThreads) * j;
+        if (y < y1) {
+            c.lam[y] = l[j];
+            const c64 p = c.phi[y];
+            acc += p.re * l[j].re + p.im * l[j].im;
+        }
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -1133,25 +1153,45 @@ k_pauli(const c64* __restrict__ psi, c64* __restrict__ lam, uint64_t dim,
 {
     __shared__ double red[256];
     double acc = 0.0;
-    for (uint64_t y = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; y < dim;
-         y += uint64_t(gridDim.x) * blockDim.x) {
-        c64 l{0.0, 0.0};
+    // blocks of kPauliPerThread x 256 amplitudes, grid-strided; each thread
+    // keeps its kPauliPerThread amplitudes in registers with the groups and
+    // terms outermost, so each term is read once per block, not per amplitude
+    constexpr int M = kPauliPerThread;
+    for (uint64_t yb = uint64_t(blockIdx.x) * blockDim.x * M; yb < dim; yb += uint64_t(gridDim.x) * blockDim.x * M) {
+        c64 l[M];
+#pragma unroll
+        for (int j = 0; j < M; j++) l[j] = c64{0.0, 0.0};
         for (uint32_t g = 0; g < ngroups; g++) {
             const PauliGroup G = groups[g];
-            const uint64_t x = y ^ G.xmask;
-            double fr = 0.0, fi = 0.0;
+            double fr[M], fi[M];
+#pragma unroll
+            for (int j = 0; j < M; j++) fr[j] = fi[j] = 0.0;
             for (uint32_t t = G.begin; t < G.end; t++) {
                 const PauliTerm T = terms[t];
-                const bool neg = __popcll(x & T.zmask) & 1;
-                fr += neg ? -T.cre : T.cre;
-                fi += neg ? -T.cim : T.cim;
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    const uint64_t x = (yb + threadIdx.x + uint64_t(blockDim.x) * j) ^ G.xmask;
+                    const bool neg = __popcll(x & T.zmask) & 1;
+                    fr[j] += neg ? -T.cre : T.cre;
+                    fi[j] += neg ? -T.cim : T.cim;
+                }
             }
-            l = cmac(l, c64{fr, fi}, psi[x]);
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                const uint64_t y = yb + threadIdx.x + uint64_t(blockDim.x) * j;
+                if (y < dim) l[j] = cmac(l[j], c64{fr[j], fi[j]}, psi[y ^ G.xmask]);
+            }
         }
-        if (lam) lam[y] = l;
-        const c64 p = psi[y];
-        if (copy) copy[y] = p;
-        acc += p.re * l.re + p.im * l.im;
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            const uint64_t y = yb + threadIdx.x + uint64_t(blockDim.x) * j;
+            if (y < dim) {
+                if (lam) lam[y] = l[j];
+                const c64 p = psi[y];
+                if (copy) copy[y] = p;
+                acc += p.re * l[j].re + p.im * l[j].im;
+            }
+        }
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -1162,7 +1202,11 @@ k_pauli(const c64* __restrict__ psi, c64* __restrict__ lam, uint64_t dim,
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-uint32_t pauli_grid(uint64_t dim) { return grid_for(dim, 256) > 2048 ? 2048 : grid_for(dim, 256); }
+uint32_t pauli_grid(uint64_t dim)
+{
+    const uint64_t g = grid_for(dim, 256 * kPauliPerThread);
+    return g > 2048 ? 2048 : uint32_t(g);
+}
 
 hipError_t launch_pauli(hipStream_t st, const c64* psi, c64* lam, uint64_t dim, const PauliGroup* groups,
                         uint32_t ngroups, const PauliTerm* terms, double* partials, c64* copy)

@@ -54,6 +54,8 @@ struct BatchArgs {
 };
 
 constexpr uint32_t kPauliChunk = 4096;     // amplitudes per Pauli task
+static_assert(kPauliChunk % kThreads == 0, "a Pauli task is whole registers of its 256 threads");
+constexpr int kPauliPerThread = 4;         // amplitudes per thread per k_pauli block
 
 
 // One launch of a batch: the steps [step_begin, step_end) of one level,
