@@ -1665,7 +1665,8 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
                 if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG)
                     gsum_tile[op.h.grad] += exec_grad_op<kRegBits, kRegsAdj>(op.h, &op, a, coef, goff, off, base);
                 else
-                    exec_state_op(op.h, &op, a, coef, goff, g, off, base, xf.data());
+                    exec_state_op(op.h, &op, a, coef, goff, g, off, base, xf.data(), sw.exthi,
+                                  uint32_t(extract_bits(base, sw.exthi)));
             }
             for (int r = 0; r < kRegs; r++) t[(goff | off[r]) ^ ph.sxor] = a[r];
         }

@@ -325,9 +325,9 @@ SVMI_HD inline c64 tile_factor(const DOpHdr& h, const c64* __restrict__ coef, ui
 // The four {u, l} pairs are read together (one scalar load) and each slot
 // is one uniform branch.
 template <int N>
-SVMI_HD inline void apply_rot(const DOpHdr& h, c64 (&a)[N], const c64* __restrict__ coef, uint64_t base)
+SVMI_HD inline void apply_rot(const DOpHdr& h, c64 (&a)[N], const c64* __restrict__ coef, uint32_t v)
 {
-    if (h.s0) apply_rtab<N>(a, coef + h.grad + 16u * rtab_variant(h.emask, base));
+    if (h.s0) apply_rtab<N>(a, coef + h.grad + 16u * v);
     const c64* L = coef + h.coef;
     const c64 f0 = L[0], f1 = L[1], f2 = L[2], f3 = L[3];
 #define SVMI_ROT_SLOT(S, F)                                              \
@@ -351,16 +351,23 @@ SVMI_HD inline void apply_rot(const DOpHdr& h, c64 (&a)[N], const c64* __restric
 
 // h = the op's header (prefetched), op = the op in memory (diagonal tail),
 // grp = the group index (goff with the register bits squeezed out).
+// vmask / vhi: the sweep's variant qubits (SweepInfo::exthi) and the tile
+// base's bits there, so a table over all of them finds its variant with one
+// compare (otherwise the bits of its own mask are extracted).
 template <int kMaxQ = kRegBits, int N = kRegs>
 SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c64 (&a)[N],
                                   const c64* __restrict__ coef, uint32_t goff, uint32_t grp,
-                                  const uint32_t (&off)[kRegs], uint64_t base, const c64* xf)
+                                  const uint32_t (&off)[kRegs], uint64_t base, const c64* xf,
+                                  uint64_t vmask = 0, uint32_t vhi = 0)
 {
+    const auto variant = [&](uint64_t emask) {
+        return emask == 0 ? 0u : (emask == vmask ? vhi : rtab_variant(emask, base));
+    };
     // the op kinds of rotation/table phases first (one or two compares),
     // then the rest
     if (h.type == DOP_ROT) {
         SVMI_FENCE(DOP_ROT + 100);
-        apply_rot<N>(h, a, coef, base);
+        apply_rot<N>(h, a, coef, h.s0 ? variant(h.emask) : 0u);
         SVMI_FENCE(DOP_ROT + 100);
         return;
     }
@@ -372,7 +379,7 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
     }
     if (h.type == DOP_RTAB) {
         SVMI_FENCE(DOP_RTAB + 100);
-        apply_rtab<N>(a, coef + h.coef + 16u * rtab_variant(h.emask, base));
+        apply_rtab<N>(a, coef + h.coef + 16u * variant(h.emask));
         SVMI_FENCE(DOP_RTAB + 100);
         return;
     }

@@ -100,6 +100,7 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
 {
     const bool active = !kIdle || tid < nt;
     const uint32_t nwaves = (nt + 63) / 64;
+    const uint32_t vhi = uint32_t(extract_bits(base, sw.exthi));
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
         if (active) {
             const PhaseInfo ph = phases[p];
@@ -138,7 +139,7 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
                         continue;
                     }
                 }
-                exec_state_op<kMaxQ>(h, op, a, coef, goff, tid, off, base, xf);
+                exec_state_op<kMaxQ>(h, op, a, coef, goff, tid, off, base, xf, sw.exthi, vhi);
             }
 #pragma unroll
             for (int r = 0; r < kRegs; r++) t0[lds_slot((goff | off[r]) ^ ph.sxor)] = a[r];
