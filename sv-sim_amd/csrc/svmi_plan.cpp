@@ -794,6 +794,42 @@ uint32_t quarter_turns(std::vector<PEmit>& out, long* saved)
     return f;
 }
 
+// Two-state phases run every rotation as three shears, because a gradient op
+// between a DOP_LIFT2 and the table that completes it would read the pair
+// scaled by diag(1/c, c).  Where the next op after a three-shear rotation
+// with c >= s is that table, or only lifts on other slots lead to it (no
+// gradient op, nothing else on the slot), the rotation runs as two shears
+// and the table takes diag(c, 1/c) on its slot.  Returns the conversions
+// (one FMA per amplitude each).
+long two_state_lifts(std::vector<PEmit>& out)
+{
+    long conv = 0;
+    for (size_t j = 0; j < out.size(); j++) {
+        if (out[j].s || out[j].d.h.type != DOP_LIFT3) continue;
+        const uint32_t slot = out[j].d.h.s0;
+        size_t k = j + 1;
+        while (k < out.size() && !out[k].s && (out[k].d.h.type == DOP_LIFT2 || out[k].d.h.type == DOP_LIFT3) &&
+               out[k].d.h.s0 != slot)
+            k++;
+        if (k == out.size() || out[k].s || out[k].d.h.type != DOP_RTAB) continue;
+        DOpHdr& h = out[j].d.h;
+        const double u = lift_u(h), l = lift_l(h);
+        const double sn = l, cs = 1.0 + u * l;   // u = (c - 1)/s, l = s
+        if (!(cs >= sn) || !(cs > 0.0)) continue;
+        const double u2 = -sn / cs, l2 = sn * cs;
+        h.type = DOP_LIFT2;
+        lift_set(h, u2, l2);
+        out[j].coef.assign(1, c64{u2, l2});
+        std::vector<c64>& T = out[k].coef;
+        for (size_t e = 0; e < T.size(); e++) {
+            const double f = ((e >> slot) & 1u) ? 1.0 / cs : cs;
+            T[e] = c64{T[e].re * f, T[e].im * f};
+        }
+        conv++;
+    }
+    return conv;
+}
+
 // Compile the phase's op list; returns the cost of the result.  two_state:
 // an adjoint phase (every state op acts on phi and lambda, gradient ops read
 // both).  Its rotations all use three shears: DOP_LIFT2 leaves the state
@@ -978,6 +1014,8 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
         long saved = 0;
         *flip = quarter_turns(out, &saved);
         cost -= saved;
+    } else {
+        cost -= two_state_lifts(out);
     }
     // runs of [table] + lifts on distinct slots become one DOP_ROT (one
     // dispatch instead of one per op): coef = the four {u, l} pairs (zero for
