@@ -1634,7 +1634,12 @@ std::string describe_program(const Program& prog)
             ss << "]\n";
             for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
                 const PhaseInfo& ph = prog.phases[p];
-                ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3] << " :";
+                ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3];
+                bool plain_gather = ph.gxor == 0;
+                for (int b = 0; b < kRegBits; b++) plain_gather = plain_gather && ph.gcol[b] == (1u << ph.r[b]);
+                if (!plain_gather) ss << " [g]";   // leading permutations folded into the gather
+                if (ph.sxor) ss << " [q]";         // quarter turns: scatter map r ^ f
+                ss << " :";
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                     const DOp& d = prog.ops[o];
                     static const char* nm[] = {"-", "D", "X", "CX", "SW", "SRN", "P", "G", "gD", "gG", "PS", "CD",

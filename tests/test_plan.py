@@ -190,3 +190,27 @@ def test_batch_tuning_table_and_overrides():
     with pytest.raises(svmi.SvmiError) as e:
         svmi.batch_adjoint([dict(items[0], adj_tile=12)], _emulate=True)
     assert "adj_tile" in str(e.value)
+
+
+@pytest.mark.parametrize("tile", [6, 8, 12])
+def test_register_coordinate_rewrites_match_oracle(tile):
+    """Quarter turns kept in register coordinates ([q] phases: three-shear
+    rotations as two shears, later tables, lift scales, signs and controlled
+    phases rewritten, scatter map XORed) and leading permutations folded into
+    gather maps ([g] phases), in circuits whose plans use them, against the
+    oracle; the adjoint's folded CX ladders against parameter shift."""
+    n = 13
+    for gates in (W.u3_brickwall(n, 10), W.random_layers(n, 30)):
+        st = W.random_state(n, seed=tile)
+        assert _err(svmi.emulate(n, gates, st, tile_qubits=tile), O.run(n, gates, st)) < 1e-12
+    assert "[q]" in svmi.plan(n, W.u3_brickwall(n, 10), tile_qubits=tile)[1]
+    assert "[g]" in svmi.plan(n, W.random_layers(n, 30), tile_qubits=tile)[1]
+    rng = np.random.default_rng(tile)
+    gates, slots, P = W.hea(n, 3, rng)
+    terms = W.ising_terms(n, rng)
+    adj_tile = min(tile, 11)
+    assert "[g]" in svmi.plan_adjoint(n, gates, slots, P, tile_qubits=adj_tile)[1]
+    st = W.random_state(n, seed=tile + 1)
+    E, g = svmi.emulate_adjoint(n, gates, terms, slots, P, st, tile_qubits=adj_tile)
+    assert abs(E - O.expval(n, *O.run(n, gates, st), terms)) < 1e-12
+    assert np.abs(g - O.param_shift_grad(n, gates, slots, P, terms, state=st)).max() < 1e-12
