@@ -65,7 +65,8 @@ svmi_status svmi_debug_plan_qir(int n, const svmi_qir_gate* gates, size_t n_gate
 /* Source of the specialised sweep kernels (SVMI_OPT_SPECIALIZE) for a
  * circuit (buf, truncated to buflen-1 bytes; may be NULL) and, if compile
  * != 0, its hipRTC compilation for gfx950 (no device needed): time and
- * code-object size. */
+ * code-object size.  compile = 2: with SVMI_OPT_PREFETCH's persistent
+ * double-buffered kernels, sized for a 256-CU device. */
 svmi_status svmi_debug_jit(int n, const svmi_gate* gates, size_t n_gates, int tile_qubits, int low_qubits,
                            int compile, double* compile_ms, uint64_t* code_bytes, char* buf, size_t buflen);
 

@@ -1044,7 +1044,13 @@ svmi_status install_program(svmi_handle h)
     h->stats.specialized_sweeps = 0;
     h->stats.compile_ms = 0.0;
     if (!h->specialize) return SVMI_OK;
-    if (!jit_build(h->prog, device_arch().c_str(), h->jit, h->jit_error)) {
+    uint64_t dma_grid = 0;   // SVMI_OPT_PREFETCH: double-buffered kernels, one workgroup per CU
+    if (h->prefetch) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) == hipSuccess && cus > 0)
+            dma_grid = uint64_t(cus);
+    }
+    if (!jit_build(h->prog, device_arch().c_str(), h->jit, h->jit_error, dma_grid)) {
         h->jit.release();
         // the sweeps are interpreted; the reason stays readable
         if (!h->jit_error.empty()) g_err = "specialisation: " + h->jit_error;
@@ -1923,7 +1929,8 @@ svmi_status svmi_debug_jit(int n, const svmi_gate* gates, size_t n_gates, int ti
     Program prog;
     plan_forward(gates, n_gates, std::max(n, kRegBits), debug_options(1, tile_qubits, low_qubits, 0), prog);
     std::vector<char> covered;
-    const std::string src = jit_source(prog, covered);
+    const uint64_t dma_grid = compile == 2 ? 256 : 0;   // double-buffered kernels for a 256-CU device
+    const std::string src = jit_source(prog, covered, dma_grid);
     if (buf && buflen) {
         const size_t m = std::min(buflen - 1, src.size());
         std::memcpy(buf, src.data(), m);
@@ -1932,7 +1939,7 @@ svmi_status svmi_debug_jit(int n, const svmi_gate* gates, size_t n_gates, int ti
     if (compile) {
         std::string err;
         double ms = 0.0;
-        const size_t bytes = jit_compile_only(prog, "gfx950", &ms, err);
+        const size_t bytes = jit_compile_only(prog, "gfx950", &ms, err, dma_grid);
         if (!bytes) return fail(SVMI_ERR_UNSUPPORTED, "jit: " + err);
         if (compile_ms) *compile_ms = ms;
         if (code_bytes) *code_bytes = bytes;

@@ -474,4 +474,53 @@ SVMI_HD inline uint64_t tile_base(const SweepInfo& sw, uint64_t tile)
     return deposit_bits(tile, lo) | deposit_bits(tile >> __builtin_popcountll(lo), sw.exthi);
 }
 
+#if defined(__HIPCC__)
+// ------------------------------------------------------------ LDS DMA
+//
+// gfx950's LDS DMA (global_load_lds_dwordx4) moves 16 bytes per lane from
+// global memory straight into LDS, lane i of a wave at M0 + 16 i, without
+// holding registers.  It is issued from inline asm: the compiler cannot tell
+// the two tile buffers apart and would wait for the prefetch before every
+// LDS read of the other buffer; the sweep waits for it explicitly (vmcnt,
+// which gfx9-family GPUs decrement in issue order for loads and stores
+// alike) before the barrier that publishes the buffer.
+__device__ __attribute__((always_inline)) inline uint32_t lds_address(const void* p)
+{
+    return uint32_t(reinterpret_cast<unsigned long long>((__attribute__((address_space(3))) const void*)p));
+}
+
+__device__ __attribute__((always_inline)) inline void lds_dma16(const c64* src, uint32_t lds_byte)
+{
+    asm volatile("s_mov_b32 m0, %1\n\t"
+                 "s_nop 0\n\t"
+                 "global_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(src), "s"(lds_byte)
+                 : "memory", "m0");
+}
+
+
+// Issue the DMA of the tile at `base` into the buffer at LDS byte address
+// lds_buf: the j-th DMA of wave w fills slots [64 (4 j + w), +64); slot S
+// holds tile-local index lds_slot(S) (the swizzle is an involution), whose
+// global index is base | ht[l >> c] | (l & cmask).  All addresses are formed
+// before the first DMA, so no LDS read waits behind one.
+__device__ __attribute__((always_inline)) inline void
+tile_dma(const c64* __restrict__ s, const uint64_t* ht, uint32_t c, uint64_t base, uint32_t lds_buf,
+         uint32_t wave, uint32_t lane)
+{
+    const uint32_t cmask = (1u << c) - 1u;
+    const c64* src[kDmaPerWave];
+#pragma unroll
+    for (uint32_t j = 0; j < kDmaPerWave; j++) {
+        const uint32_t l = lds_slot((j * kDmaWaves + wave) * 64u + lane);
+        src[j] = s + (base | ht[l >> c] | (l & cmask));
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kDmaPerWave; j++)
+        lds_dma16(src[j], __builtin_amdgcn_readfirstlane(lds_buf + (j * kDmaWaves + wave) * 64u * 16u));
+}
+
+#endif
+
 }  // namespace svmi

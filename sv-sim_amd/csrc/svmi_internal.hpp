@@ -91,6 +91,11 @@ SVMI_HD inline uint32_t sweep_threads(uint32_t k, int ns) { return 1u << (k - (n
 constexpr int kMaxDenseQubitsAdj = 2;       // two-state sweeps run dense ops of <= 2 qubits
                                             // (wider gates: full-state passes on both states)
 constexpr int kThreads = 256;               // threads of the reduction / Pauli workgroups
+// double-buffered sweeps (LDS DMA of the next tile): 12-qubit tiles, 256 threads
+constexpr uint32_t kDmaTileQubits = 12;
+constexpr uint32_t kDmaThreads = 1u << (kDmaTileQubits - kRegBits);      // 256
+constexpr uint32_t kDmaWaves = kDmaThreads / 64;                          // 4
+constexpr uint32_t kDmaPerWave = (1u << kDmaTileQubits) / 64 / kDmaWaves; // 16 DMA per wave per tile
 constexpr int kMaxTileThreads = 1 << (kMaxTileQubits - kRegBits);   // 512
 constexpr int kMaxDenseQubits = 5;          // standalone dense ops
 constexpr int kMaxGradsPerSweep = 256;      // adjoint accumulators per sweep (LDS)

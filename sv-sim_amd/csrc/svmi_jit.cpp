@@ -136,45 +136,39 @@ void emit_state_op(std::ostringstream& o, const DOp& d, const PhaseInfo& ph)
     }
 }
 
-void emit_sweep(std::ostringstream& o, const Program& prog, uint32_t i)
+// `var` = the tile base of tile id `tile` (the id's bits deposited on the
+// qubits outside the tile).
+void emit_base(std::ostringstream& o, const SweepInfo& sw, const char* var, const char* tile, const char* ind)
 {
-    const SweepInfo& sw = prog.sweeps[i];
-    const uint32_t nt = 1u << (sw.k - kRegBits);
-    // second bound: at least two waves per SIMD, i.e. at most 256 VGPRs
-    o << "extern \"C\" __global__ void __launch_bounds__(" << nt << ", 2)\n"
-      << "svmi_jit_" << i << "(c64* __restrict__ s, const c64* __restrict__ coef, const uint64_t* __restrict__ ht, "
-         "uint64_t tile0)\n{\n"
-      << "    __shared__ __align__(16) c64 t0[" << (1u << sw.k) << "];\n"
-      << "    __shared__ __align__(16) c64 xf[" << (sw.n_xf ? sw.n_xf : 1u) << "];\n"
-      << "    const uint32_t tid = threadIdx.x;\n"
-      << "    const uint64_t tile = tile0 + blockIdx.x;\n"
-      << "    uint64_t base = 0;\n";
-    // tile base: the tile id's bits deposited on the qubits outside the tile
+    o << ind << "uint64_t " << var << " = 0;\n";
     int j = 0;
     for (const uint64_t m : {sw.extmask & ~sw.exthi, sw.exthi})
         for (int p = 0; p < 64; p++)
-            if ((m >> p) & 1) o << "    base |= ((tile >> " << j++ << ") & 1ull) << " << p << ";\n";
-    const uint32_t cmask = (1u << sw.c) - 1u;
-    o << "#pragma unroll\n    for (int j = 0; j < " << kRegs << "; j++) {\n"
-      << "        const uint32_t l = tid + " << nt << "u * j;\n"
-      << "        t0[lds_slot(l)] = s[base | ht[" << sw.hitab << "u + (l >> " << sw.c << ")] | (l & " << cmask
-      << "u)];\n    }\n";
-    // tile-uniform phase factors
+            if ((m >> p) & 1) o << ind << var << " |= ((" << tile << " >> " << j++ << ") & 1ull) << " << p << ";\n";
+}
+
+// The tile's phase factors into xf (uses `base`, `tid`).
+void emit_factors(std::ostringstream& o, const Program& prog, const SweepInfo& sw, uint32_t nt, const char* ind)
+{
     if (sw.n_xf) {
-        o << "    uint64_t xbits = 0;\n";
+        o << ind << "uint64_t xbits = 0;\n";
         int jx = 0;
         for (int p = 0; p < 64; p++)
-            if ((sw.extmask >> p) & 1) o << "    xbits |= ((base >> " << p << ") & 1ull) << " << jx++ << ";\n";
+            if ((sw.extmask >> p) & 1) o << ind << "xbits |= ((base >> " << p << ") & 1ull) << " << jx++ << ";\n";
     }
     for (uint32_t op = sw.op_begin; op < sw.op_end; op++) {
         const DOpHdr& h = prog.ops[op].h;
         if (has_tile_factor(h)) {
-            o << "    if (tid == " << (h.grad % nt) << "u) {\n        constexpr DOpHdr H = ";
+            o << ind << "if (tid == " << (h.grad % nt) << "u) {\n" << ind << "    constexpr DOpHdr H = ";
             emit_hdr(o, h);
-            o << ";\n        xf[" << h.grad << "] = tile_factor(H, coef, base, xbits);\n    }\n";
+            o << ";\n" << ind << "    xf[" << h.grad << "] = tile_factor(H, coef, base, xbits);\n" << ind << "}\n";
         }
     }
-    o << "    __syncthreads();\n";
+}
+
+// The sweep's phases on the tile at t0 (each ends with a barrier).
+void emit_phases(std::ostringstream& o, const Program& prog, const SweepInfo& sw)
+{
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
         const PhaseInfo& ph = prog.phases[p];
         o << "    {\n        constexpr PhaseInfo P = {{" << ph.r[0] << "u, " << ph.r[1] << "u, " << ph.r[2] << "u, "
@@ -213,10 +207,79 @@ void emit_sweep(std::ostringstream& o, const Program& prog, uint32_t i)
           << "u)] = a[r];\n"
           << "    }\n    __syncthreads();\n";
     }
+}
+
+void emit_sweep(std::ostringstream& o, const Program& prog, uint32_t i)
+{
+    const SweepInfo& sw = prog.sweeps[i];
+    const uint32_t nt = 1u << (sw.k - kRegBits);
+    // second bound: at least two waves per SIMD, i.e. at most 256 VGPRs
+    o << "extern \"C\" __global__ void __launch_bounds__(" << nt << ", 2)\n"
+      << "svmi_jit_" << i << "(c64* __restrict__ s, const c64* __restrict__ coef, const uint64_t* __restrict__ ht, "
+         "uint64_t tile0)\n{\n"
+      << "    __shared__ __align__(16) c64 t0[" << (1u << sw.k) << "];\n"
+      << "    __shared__ __align__(16) c64 xf[" << (sw.n_xf ? sw.n_xf : 1u) << "];\n"
+      << "    const uint32_t tid = threadIdx.x;\n"
+      << "    const uint64_t tile = tile0 + blockIdx.x;\n";
+    emit_base(o, sw, "base", "tile", "    ");
+    const uint32_t cmask = (1u << sw.c) - 1u;
+    o << "#pragma unroll\n    for (int j = 0; j < " << kRegs << "; j++) {\n"
+      << "        const uint32_t l = tid + " << nt << "u * j;\n"
+      << "        t0[lds_slot(l)] = s[base | ht[" << sw.hitab << "u + (l >> " << sw.c << ")] | (l & " << cmask
+      << "u)];\n    }\n";
+    emit_factors(o, prog, sw, nt, "    ");
+    o << "    __syncthreads();\n";
+    emit_phases(o, prog, sw);
     o << "#pragma unroll\n    for (int j = 0; j < " << kRegs << "; j++) {\n"
       << "        const uint32_t l = tid + " << nt << "u * j;\n"
       << "        s[base | ht[" << sw.hitab << "u + (l >> " << sw.c << ")] | (l & " << cmask
       << "u)] = t0[lds_slot(l)];\n    }\n}\n\n";
+}
+
+// Sweeps of 12-qubit tiles as persistent double-buffered kernels (one
+// workgroup of 256 threads per CU): while the phases of tile t run in one LDS
+// buffer, tile t + gridDim.x streams into the other by LDS DMA (tile_dma);
+// the wait for a tile's DMA leaves only the previous tile's 16 stores per
+// thread outstanding (vmcnt(16); the first tile waits for everything).
+bool dma_eligible(const SweepInfo& sw) { return sw.k == kDmaTileQubits && sw.c <= kDmaTileQubits; }
+
+void emit_sweep_dma(std::ostringstream& o, const Program& prog, uint32_t i)
+{
+    const SweepInfo& sw = prog.sweeps[i];
+    const uint32_t nt = kDmaThreads, kt = 1u << kDmaTileQubits, rows = kt >> sw.c;
+    const uint32_t cmask = (1u << sw.c) - 1u;
+    o << "extern \"C\" __global__ void __launch_bounds__(" << nt << ", 1)\n"
+      << "svmi_jit_" << i << "(c64* __restrict__ s, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab, "
+         "uint64_t n_tiles)\n{\n"
+      << "    __shared__ __align__(16) c64 tb[" << 2 * kt << "];\n"
+      << "    __shared__ __align__(16) uint64_t ht[" << rows << "];\n"
+      << "    __shared__ __align__(16) c64 xf[" << (sw.n_xf ? sw.n_xf : 1u) << "];\n"
+      << "    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;\n"
+      << "    for (uint32_t i = tid; i < " << rows << "u; i += " << nt << "u) ht[i] = hitab[" << sw.hitab << "u + i];\n"
+      << "    __syncthreads();\n"
+      << "    const uint32_t lds0 = lds_address(tb);\n"
+      << "    uint64_t t = blockIdx.x;\n"
+      << "    {\n";
+    emit_base(o, sw, "b0", "t", "        ");
+    o << "        tile_dma(s, ht, " << sw.c << "u, b0, lds0, wave, lane);\n    }\n"
+      << "    uint32_t cur = 0;\n    bool first = true;\n    for (;;) {\n"
+      << "        if (first) asm volatile(\"s_waitcnt vmcnt(0)\" ::: \"memory\");\n"
+      << "        else asm volatile(\"s_waitcnt vmcnt(16)\" ::: \"memory\");\n"
+      << "        __syncthreads();\n";
+    emit_base(o, sw, "base", "t", "        ");
+    o << "        const uint64_t tn = t + gridDim.x;\n        const bool more = tn < n_tiles;\n";
+    emit_factors(o, prog, sw, nt, "        ");
+    o << "        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): no load of the factors still in flight\n"
+      << "        if (more) {\n";
+    emit_base(o, sw, "bn", "tn", "            ");
+    o << "            tile_dma(s, ht, " << sw.c << "u, bn, lds0 + (cur ^ 1u) * " << kt * 16 << "u, wave, lane);\n"
+      << "        }\n        __syncthreads();\n"
+      << "        c64* t0 = tb + cur * " << kt << "u;\n";
+    emit_phases(o, prog, sw);
+    o << "#pragma unroll\n        for (int j = 0; j < " << kRegs << "; j++) {\n"
+      << "            const uint32_t l = tid + " << nt << "u * j;\n"
+      << "            s[base | ht[l >> " << sw.c << "] | (l & " << cmask << "u)] = t0[lds_slot(l)];\n        }\n"
+      << "        if (!more) break;\n        t = tn;\n        cur ^= 1u;\n        first = false;\n    }\n}\n\n";
 }
 
 struct Rtc {
@@ -264,9 +327,11 @@ void JitSweeps::release()
     if (module) hipModuleUnload(module);
     module = nullptr;
     fn.clear();
+    dma.clear();
+    dma_grid = 0;
 }
 
-std::string jit_source(const Program& prog, std::vector<char>& covered)
+std::string jit_source(const Program& prog, std::vector<char>& covered, uint64_t dma_grid)
 {
     covered.assign(prog.sweeps.size(), 0);
     std::ostringstream o;
@@ -275,25 +340,33 @@ std::string jit_source(const Program& prog, std::vector<char>& covered)
     for (uint32_t i = 0; i < prog.sweeps.size(); i++) {
         const SweepInfo& sw = prog.sweeps[i];
         if (sw.k > kJitMaxTileQubits) continue;
-        emit_sweep(o, prog, i);
-        covered[i] = 1;
+        if (dma_grid && dma_eligible(sw) && sw.n_tiles >= 2 * dma_grid) {
+            emit_sweep_dma(o, prog, i);
+            covered[i] = 2;
+        } else {
+            emit_sweep(o, prog, i);
+            covered[i] = 1;
+        }
     }
     return o.str();
 }
 
-size_t jit_compile_only(const Program& prog, const char* arch, double* ms, std::string& err)
+size_t jit_compile_only(const Program& prog, const char* arch, double* ms, std::string& err, uint64_t dma_grid)
 {
     std::vector<char> covered, code;
-    const std::string src = jit_source(prog, covered);
+    const std::string src = jit_source(prog, covered, dma_grid);
     if (!compile(src, arch, code, ms, err)) return 0;
     return code.size();
 }
 
-bool jit_build(const Program& prog, const char* arch, JitSweeps& out, std::string& err)
+bool jit_build(const Program& prog, const char* arch, JitSweeps& out, std::string& err, uint64_t dma_grid)
 {
     out.release();
     std::vector<char> covered, code;
-    const std::string src = jit_source(prog, covered);
+    const std::string src = jit_source(prog, covered, dma_grid);
+    out.dma.assign(prog.sweeps.size(), 0);
+    for (size_t i = 0; i < covered.size(); i++) out.dma[i] = covered[i] == 2;
+    out.dma_grid = dma_grid;
     bool any = false;
     for (char c : covered) any = any || c;
     if (!any) return false;
@@ -319,6 +392,12 @@ bool jit_build(const Program& prog, const char* arch, JitSweeps& out, std::strin
 hipError_t jit_launch(hipStream_t st, const JitSweeps& j, uint32_t i, const SweepInfo& sw, c64* s, const c64* coef,
                       const uint64_t* hitab)
 {
+    if (i < j.dma.size() && j.dma[i]) {   // persistent: one workgroup per CU
+        const uint32_t grid = uint32_t(sw.n_tiles < j.dma_grid ? sw.n_tiles : j.dma_grid);
+        uint64_t n_tiles = sw.n_tiles;
+        void* args[] = {&s, &coef, &hitab, &n_tiles};
+        return hipModuleLaunchKernel(j.fn[i], grid, 1, 1, kDmaThreads, 1, 1, 0, st, args, nullptr);
+    }
     const uint32_t nt = 1u << (sw.k - kRegBits);
     const uint64_t max_grid = 1ull << 30;
     for (uint64_t t0 = 0; t0 < sw.n_tiles; t0 += max_grid) {

@@ -31,21 +31,25 @@ namespace svmi {
 struct JitSweeps {
     hipModule_t module = nullptr;
     std::vector<hipFunction_t> fn;   // per sweep of the program; nullptr: run by k_sweep1
+    std::vector<char> dma;           // per sweep: 1 = persistent double-buffered kernel
+    uint64_t dma_grid = 0;           // its workgroups (one per CU)
     double compile_ms = 0.0;
     void release();
 };
 
 // HIP source of the specialised kernels (`svmi_jit_<i>` for sweep i) of a
 // forward program; sweeps the generator does not cover (two states, tiles
-// above 12 qubits) get no kernel.  covered[i] = 1 for the sweeps it emits.
-std::string jit_source(const Program& prog, std::vector<char>& covered);
+// above 12 qubits) get no kernel.  covered[i] = 1 for the sweeps it emits,
+// 2 for those emitted as persistent double-buffered kernels (dma_grid != 0:
+// sweeps of 12-qubit tiles over at least two grids of dma_grid workgroups).
+std::string jit_source(const Program& prog, std::vector<char>& covered, uint64_t dma_grid);
 
 // Compile with hipRTC for `arch` and load the module on the current device.
 // Returns false (and fills err) if anything fails; `out` is then empty.
-bool jit_build(const Program& prog, const char* arch, JitSweeps& out, std::string& err);
+bool jit_build(const Program& prog, const char* arch, JitSweeps& out, std::string& err, uint64_t dma_grid);
 
 // Compile only (no device needed); returns the code object size or 0.
-size_t jit_compile_only(const Program& prog, const char* arch, double* ms, std::string& err);
+size_t jit_compile_only(const Program& prog, const char* arch, double* ms, std::string& err, uint64_t dma_grid);
 
 // Launch sweep `i` (out.fn[i] != nullptr) over all its tiles.
 hipError_t jit_launch(hipStream_t st, const JitSweeps& j, uint32_t i, const SweepInfo& sw, c64* s, const c64* coef,

@@ -275,53 +275,7 @@ k_sweep1p(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict_
 
 // ---------------------------------------------- double-buffered sweeps
 //
-// gfx950's LDS DMA (global_load_lds_dwordx4) moves 16 bytes per lane from
-// global memory straight into LDS, lane i of a wave at M0 + 16 i, without
-// holding registers.  It is issued from inline asm: the compiler cannot tell
-// the two tile buffers apart and would wait for the prefetch before every
-// LDS read of the other buffer; the sweep waits for it explicitly (vmcnt,
-// which gfx9-family GPUs decrement in issue order for loads and stores
-// alike) before the barrier that publishes the buffer.
-__device__ __attribute__((always_inline)) inline uint32_t lds_address(const void* p)
-{
-    return uint32_t(reinterpret_cast<uintptr_t>((__attribute__((address_space(3))) const void*)p));
-}
-
-__device__ __attribute__((always_inline)) inline void lds_dma16(const c64* src, uint32_t lds_byte)
-{
-    asm volatile("s_mov_b32 m0, %1\n\t"
-                 "s_nop 0\n\t"
-                 "global_load_lds_dwordx4 %0, off"
-                 :
-                 : "v"(src), "s"(lds_byte)
-                 : "memory", "m0");
-}
-
-constexpr uint32_t kDmaTileQubits = 12;
-constexpr uint32_t kDmaThreads = 1u << (kDmaTileQubits - kRegBits);    // 256
-constexpr uint32_t kDmaWaves = kDmaThreads / 64;                        // 4
-constexpr uint32_t kDmaPerWave = (1u << kDmaTileQubits) / 64 / kDmaWaves; // 16 DMA per wave per tile
-
-// Issue the DMA of the tile at `base` into the buffer at LDS byte address
-// lds_buf: the j-th DMA of wave w fills slots [64 (4 j + w), +64); slot S
-// holds tile-local index lds_slot(S) (the swizzle is an involution), whose
-// global index is base | ht[l >> c] | (l & cmask).  All addresses are formed
-// before the first DMA, so no LDS read waits behind one.
-__device__ __attribute__((always_inline)) inline void
-tile_dma(const c64* __restrict__ s, const uint64_t* ht, uint32_t c, uint64_t base, uint32_t lds_buf,
-         uint32_t wave, uint32_t lane)
-{
-    const uint32_t cmask = (1u << c) - 1u;
-    const c64* src[kDmaPerWave];
-#pragma unroll
-    for (uint32_t j = 0; j < kDmaPerWave; j++) {
-        const uint32_t l = lds_slot((j * kDmaWaves + wave) * 64u + lane);
-        src[j] = s + (base | ht[l >> c] | (l & cmask));
-    }
-#pragma unroll
-    for (uint32_t j = 0; j < kDmaPerWave; j++)
-        lds_dma16(src[j], __builtin_amdgcn_readfirstlane(lds_buf + (j * kDmaWaves + wave) * 64u * 16u));
-}
+// LDS DMA helpers (lds_dma16, tile_dma): svmi_exec.hpp.
 
 size_t dma_sweep_smem_bytes(const SweepInfo& sw)
 {

@@ -502,7 +502,8 @@ def plan_qir(n, gates, tile_qubits=12, low_qubits=4):
 
 
 def jit_source(n, gates, tile_qubits=12, low_qubits=4, compile=False):
-    """(source, compile_ms, code_bytes) of the specialised sweep kernels."""
+    """(source, compile_ms, code_bytes) of the specialised sweep kernels
+    (compile=2: with the persistent double-buffered kernels of SVMI_OPT_PREFETCH)."""
     g = as_gates(gates)
     buf = ctypes.create_string_buffer(1 << 24)
     ms = ctypes.c_double(0.0)

@@ -77,3 +77,36 @@ def test_prefetch_runs_twice_on_one_handle():
     ore, oim = O.run(n, g1)
     ore, oim = O.run(n, g2, (ore, oim))
     assert max(np.abs(re - ore).max(), np.abs(im - oim).max()) < TOL
+
+
+def _run_spec(n, gates, prefetch, state=None):
+    with svmi.Engine(n) as e:
+        e.set_option(svmi.OPT_PREFETCH, prefetch)
+        e.set_option(svmi.OPT_SPECIALIZE, 1)
+        e.upload(gates)
+        assert e.stats()["specialized_sweeps"] > 0
+        if state is not None:
+            e.set_state(*state)
+        e.run()
+        return e.get_state()
+
+
+@pytest.mark.parametrize("n,name", [(22, "u3_brickwall"), (21, "qft"), (21, "random_layers")])
+def test_specialised_prefetch_sweeps_bit_identical_and_match_oracle(n, name):
+    """Specialised kernels in the persistent double-buffered form (SVMI_OPT_SPECIALIZE
+    with SVMI_OPT_PREFETCH) against the one-tile-per-workgroup specialised kernels
+    (same arithmetic: equal states) and the oracle."""
+    state = None
+    if name == "random_layers":
+        gates = W.random_layers(n=n, layers=12)
+    elif name == "u3_brickwall":
+        gates = W.u3_brickwall(n=n, layers=6)
+    else:
+        gates = W.qft(n)
+        state = _random_state(n, 78)
+    print("case", n, name, len(gates), "gates", flush=True)
+    re1, im1 = _run_spec(n, gates, 1, state)
+    re0, im0 = _run_spec(n, gates, 0, state)
+    assert np.array_equal(re1, re0) and np.array_equal(im1, im0)
+    ore, oim = O.run(n, gates, state)
+    assert max(np.abs(re1 - ore).max(), np.abs(im1 - oim).max()) < TOL
