@@ -112,10 +112,13 @@ typedef enum svmi_option {
                                  per circuit); 0 = interpret the sweep programs (default)     */
     SVMI_OPT_MERGE_QUBITS = 9, /* widest dense block (1..4 register qubits) that fusion may
                                  form from a run of gates inside one phase (default 2)       */
-    SVMI_OPT_PREFETCH = 10    /* 1 = one-state sweeps of 12-qubit tiles over large states run
+    SVMI_OPT_PREFETCH = 10,   /* 1 = one-state sweeps of 12-qubit tiles over large states run
                                  persistent with two LDS tile buffers, the next tile streaming
                                  in by LDS DMA during the current one's phases (one workgroup
                                  per CU); 0 = one workgroup per tile, two per CU (default)   */
+    SVMI_OPT_TILE_MAP = 11    /* adjoint sweeps: 1 = CX and SWAP gates on two tile qubits that a
+                                 phase does not hold in registers relabel the tile's LDS
+                                 indices instead of needing a phase (default); 0 = off       */
 } svmi_option;
 
 /* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,

@@ -34,6 +34,59 @@ svmi_status fail(svmi_status st, const std::string& msg)
     return st;
 }
 
+// Adds v to a GF(2) basis kept by leading bit (basis[b] has top bit b);
+// false when v is already in its span.
+bool gf2_insert(uint32_t (&basis)[32], uint32_t v)
+{
+    for (int b = 31; b >= 0 && v; b--)
+        if ((v >> b) & 1u) {
+            if (!basis[b]) {
+                basis[b] = v;
+                return true;
+            }
+            v ^= basis[b];
+        }
+    return false;
+}
+
+bool gf2_in_span(const uint32_t (&basis)[32], uint32_t v)
+{
+    for (int b = 31; b >= 0 && v; b--)
+        if ((v >> b) & 1u) {
+            if (!basis[b]) return false;
+            v ^= basis[b];
+        }
+    return v == 0;
+}
+
+// A phase's LDS addressing stays inside its tile and is a bijection there:
+// the tile map's columns (scol at the tile register bits, gm at the group
+// bits) are independent and below 2^k, the gather columns span the same
+// register columns, gxor and sxor lie in that span, and a two-state phase's
+// state column is 2^k.
+bool phase_map_valid(const PhaseInfo& ph, uint32_t k, bool two)
+{
+    const int rb = two ? kRegBitsAdj : kRegBits;
+    if (k < uint32_t(rb) || k - rb > 10) return false;
+    const uint32_t lim = 1u << k;
+    uint32_t all[32] = {}, reg[32] = {}, gat[32] = {};
+    for (int s = 0; s < rb; s++) {
+        if (ph.scol[s] >= lim || !gf2_insert(all, ph.scol[s])) return false;
+        gf2_insert(reg, ph.scol[s]);
+    }
+    for (int i = 0; i < 10; i++) {
+        if (uint32_t(i) >= k - rb) {
+            if (ph.gm[i]) return false;
+        } else if (ph.gm[i] >= lim || !gf2_insert(all, ph.gm[i])) {
+            return false;
+        }
+    }
+    for (int s = 0; s < rb; s++)
+        if (!gf2_in_span(reg, ph.gcol[s]) || !gf2_insert(gat, ph.gcol[s])) return false;
+    if (two && (ph.scol[kRegBits - 1] != lim || ph.gcol[kRegBits - 1] != lim)) return false;
+    return gf2_in_span(reg, ph.gxor) && gf2_in_span(reg, ph.sxor);
+}
+
 #define SVMI_HIP(x)                                                                          \
     do {                                                                                     \
         hipError_t e_ = (x);                                                                 \
@@ -376,6 +429,13 @@ svmi_status verify_batch(const BatchPlan& bp)
             if (sw.op_begin > sw.op_end || sw.op_end > bp.ops.size() || sw.n_xf > uint32_t(kMaxXfSlots) ||
                 16u * sw.n_lt > sw.n_xf || sw.lt_coef + 16ull * sw.n_lt > bp.coef.size())
                 return fail(SVMI_ERR_STATE, "batch plan: bad sweep op range");
+            if (sw.mapped) {   // the store's tile map: a bijection of the tile
+                uint32_t basis[32] = {};
+                bool ok = b.kind == BSTEP_ADJ;
+                for (uint32_t i = 0; i < 14 && ok; i++)
+                    ok = i < sw.k ? (sw.fcol[i] < (1u << sw.k) && gf2_insert(basis, sw.fcol[i])) : sw.fcol[i] == 0;
+                if (!ok) return fail(SVMI_ERR_STATE, "batch plan: bad tile map");
+            }
             for (uint32_t o = sw.op_begin; o < sw.op_end; o++)
                 if ((has_tile_factor(bp.ops[o].h) && bp.ops[o].h.grad >= sw.n_xf) ||
                     (bp.ops[o].h.type == DOP_PHASETAB && bp.ops[o].h.gmask != kLaneTabGlobal &&
@@ -393,12 +453,8 @@ svmi_status verify_batch(const BatchPlan& bp)
                     if (state_bit ? ph.r[r] != sw.k : ph.r[r] >= sw.k)
                         return fail(SVMI_ERR_STATE, "batch plan: bad register bit");
                 }
-                // the gather map moves amplitudes only among the register bits
-                uint32_t regmask = 0;
-                for (int r = 0; r < kRegBits; r++) regmask |= 1u << ph.r[r];
-                bool map_ok = (ph.gxor & ~regmask) == 0 && (ph.sxor & ~regmask) == 0;
-                for (int r = 0; r < kRegBits; r++) map_ok = map_ok && ph.gcol[r] && (ph.gcol[r] & ~regmask) == 0;
-                if (!map_ok) return fail(SVMI_ERR_STATE, "batch plan: bad gather map");
+                if (!phase_map_valid(ph, sw.k, two) || (!two && ph.mapped))
+                    return fail(SVMI_ERR_STATE, "batch plan: bad gather map");
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++)
                     if (bp.ops[o].h.coef > bp.coef.size()) return fail(SVMI_ERR_STATE, "batch plan: bad coef offset");
             }
@@ -1561,6 +1617,7 @@ svmi_status svmi_set_option(svmi_handle h, int option, int64_t value)
         if (value < 1 || value > kRegBits) return fail(SVMI_ERR_ARG, "merge qubits must be in [1, 4]");
         h->opt.merge_qubits = int(value);
         break;
+    case SVMI_OPT_TILE_MAP: h->opt.tile_map = value != 0; break;
     default: return fail(SVMI_ERR_ARG, "unknown option " + std::to_string(option));
     }
     h->adj_valid = false;
@@ -1662,10 +1719,13 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
         uint32_t off[kRegs], goffs[kRegs];
         phase_offsets(ph, off);
         phase_gather_offsets(ph, goffs);
+        uint32_t soffs[kRegs];
+        phase_scatter_offsets(ph, soffs);
         for (uint32_t g = 0; g < ngroups; g++) {
             const uint32_t goff = group_offset(ph, g);
+            const uint32_t gphys = ns == 2 ? mapped_group_offset(ph, g) : goff;
             c64 a[kRegs];
-            for (int r = 0; r < kRegs; r++) a[r] = t[goff | goffs[r]];
+            for (int r = 0; r < kRegs; r++) a[r] = t[gphys ^ goffs[r]];
             for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
                 const DOp& op = ops[o];
                 if (op.h.type == DOP_GRAD_DENSE || op.h.type == DOP_GRAD_DIAG)
@@ -1674,12 +1734,13 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
                     exec_state_op(op.h, &op, a, coef, goff, g, off, base, xf.data(), sw.exthi,
                                   uint32_t(extract_bits(base, sw.exthi)));
             }
-            for (int r = 0; r < kRegs; r++) t[(goff | off[r]) ^ ph.sxor] = a[r];
+            for (int r = 0; r < kRegs; r++) t[ns == 2 ? (gphys ^ soffs[r]) : ((goff | off[r]) ^ ph.sxor)] = a[r];
         }
     }
     for (uint32_t l = 0; l < tsize; l++) {
-        s0[gidx[l]] = t[l];
-        if (ns == 2) s1[gidx[l]] = t[tsize + l];
+        const uint32_t m = ns == 2 && sw.mapped ? sweep_map(sw, l) : l;
+        s0[gidx[l]] = t[m];
+        if (ns == 2) s1[gidx[l]] = t[tsize + m];
     }
 }
 

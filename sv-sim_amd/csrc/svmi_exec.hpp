@@ -451,6 +451,41 @@ SVMI_HD inline void phase_gather_offsets(const PhaseInfo& ph, uint32_t (&off)[kR
     }
 }
 
+// Scatter offsets of a mapped phase: register rho is written to LDS offset
+// sxor ^ (XOR of scol[s] over the set bits s of rho) within the group.
+SVMI_HD inline void phase_scatter_offsets(const PhaseInfo& ph, uint32_t (&off)[kRegs])
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++) {
+        uint32_t o = ph.sxor;
+#pragma unroll
+        for (int s = 0; s < kRegBits; s++)
+            if ((r >> s) & 1) o ^= ph.scol[s];
+        off[r] = o;
+    }
+}
+
+// LDS index of group g's base under a phase's tile map: XOR of gm[i] over the
+// set bits i of g (the group's register bits being zero).
+SVMI_HD inline uint32_t mapped_group_offset(const PhaseInfo& ph, uint32_t g)
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < 10; i++)
+        if ((g >> i) & 1u) o ^= ph.gm[i];
+    return o;
+}
+
+// LDS index of tile-local index l under a sweep's final tile map.
+SVMI_HD inline uint32_t sweep_map(const SweepInfo& sw, uint32_t l)
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < 14; b++)
+        if ((l >> b) & 1u) o ^= sw.fcol[b];
+    return o;
+}
+
 // Tile-local base index of group g: insert zero bits at the 4 register bits.
 SVMI_HD inline uint32_t group_offset(const PhaseInfo& ph, uint32_t g)
 {

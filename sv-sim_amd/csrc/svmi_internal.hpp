@@ -232,8 +232,17 @@ struct PhaseInfo {
     // scatter map: register rho is written to tile-local offset off[rho] ^ sxor
     // (the quarter turns a phase keeps in its register coordinates)
     uint32_t sxor;
+    // Tile map (two-state sweeps): the amplitude of tile-local index x sits at
+    // LDS index M(x), M linear over GF(2) (tile-local CX and SWAP ops folded
+    // into the addressing instead of run).  scol[s] = M(1 << r[s]) and gm[i] =
+    // M of the group number's bit i (the i-th non-register tile bit); gcol,
+    // gxor and sxor above are then LDS offsets in the span of scol.  One-state
+    // phases keep M the identity.
+    uint32_t scol[4];
+    uint16_t gm[10];
+    uint32_t mapped;     // M differs from the identity
 };
-static_assert(sizeof(PhaseInfo) == 48, "PhaseInfo layout");
+static_assert(sizeof(PhaseInfo) == 88, "PhaseInfo layout");
 
 struct SweepInfo {
     uint32_t phase_begin, phase_end;
@@ -257,6 +266,8 @@ struct SweepInfo {
                          // with max_dense <= 2 run a kernel compiled without wider blocks
     uint32_t lt_coef;    // coef offset of a contiguous copy of those lane tables (16 c64 each),
     uint32_t n_lt;       // which every tile copies into the end of its factor area
+    uint32_t mapped;     // two-state sweeps: the tile map after the last phase is not the
+    uint16_t fcol[14];   // identity; fcol[b] = M(1 << b), read by the tile's store
 };
 
 // A standalone op (run by its own kernel on the whole state, outside sweeps).

@@ -48,6 +48,22 @@ tile_put(const c64 (&r)[J], c64* t, uint32_t lo, uint32_t tid, uint32_t nt)
     for (int j = 0; j < J; j++) t[lds_slot(lo + tid + nt * j)] = r[j];
 }
 
+// LDS -> HBM for the tile at `base` when the sweep ends with a tile map:
+// tile-local index l = tid + nt j sits at LDS index M(tid) ^ M(nt j).
+template <int J>
+__device__ __attribute__((always_inline)) inline void
+tile_store_mapped(const c64* t, uint32_t lo, c64* __restrict__ s, const SweepInfo& sw,
+                  const uint64_t* __restrict__ ht, uint64_t base, uint32_t tid, uint32_t nt)
+{
+    const uint32_t cmask = (1u << sw.c) - 1u;
+    const uint32_t mt = sweep_map(sw, tid);
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const uint32_t l = tid + nt * j;
+        s[base | ht[l >> sw.c] | (l & cmask)] = t[lds_slot(lo + (mt ^ sweep_map(sw, nt * j)))];
+    }
+}
+
 // LDS -> HBM for the tile at `base`.
 template <int J>
 __device__ __attribute__((always_inline)) inline void
@@ -110,10 +126,14 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
             const uint32_t goff = group_offset(ph, tid);
             // NS = 2: the phase's last register bit is k, so a[0..7] are
             // phi's amplitudes and a[8..15] lambda's (t1 = t0 + 2^k) and every
-            // op, dispatched once, acts on both
+            // op, dispatched once, acts on both.  The group's amplitudes sit
+            // at LDS indices under the tile map (gphys, gather and scatter
+            // offsets); the ops see the plain tile-local indices (goff, off).
+            uint32_t gphys = goff;
+            if constexpr (NS == 2) gphys = mapped_group_offset(ph, tid);
             c64 a[kRegs];
 #pragma unroll
-            for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(goff | goffs[r])];
+            for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(NS == 2 ? (gphys ^ goffs[r]) : (goff | goffs[r]))];
             // op headers are fetched one op ahead so the scalar-load latency
             // of the next op overlaps the arithmetic of the current one (every
             // op array ends with a NOP, so the read past a phase's last op
@@ -141,8 +161,15 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
                 }
                 exec_state_op<kMaxQ>(h, op, a, coef, goff, tid, off, base, xf, sw.exthi, vhi);
             }
+            if constexpr (NS == 2) {
+                uint32_t soffs[kRegs];
+                phase_scatter_offsets(ph, soffs);
 #pragma unroll
-            for (int r = 0; r < kRegs; r++) t0[lds_slot((goff | off[r]) ^ ph.sxor)] = a[r];
+                for (int r = 0; r < kRegs; r++) t0[lds_slot(gphys ^ soffs[r])] = a[r];
+            } else {
+#pragma unroll
+                for (int r = 0; r < kRegs; r++) t0[lds_slot((goff | off[r]) ^ ph.sxor)] = a[r];
+            }
         }
         __syncthreads();
     }
@@ -202,8 +229,13 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
     __syncthreads();
     tile_phases<NS, kIdle, kMaxQ>(t0, gacc, xf, sw, phases, ops, coef, base, tid, nt);
     if (active) {
-        tile_store<J>(t0, 0u, s0, sw, ht, base, tid, nt);
-        if constexpr (NS == 2) tile_store<J>(t0, tsize, s1, sw, ht, base, tid, nt);
+        if (NS == 2 && sw.mapped) {
+            tile_store_mapped<J>(t0, 0u, s0, sw, ht, base, tid, nt);
+            tile_store_mapped<J>(t0, tsize, s1, sw, ht, base, tid, nt);
+        } else {
+            tile_store<J>(t0, 0u, s0, sw, ht, base, tid, nt);
+            if constexpr (NS == 2) tile_store<J>(t0, tsize, s1, sw, ht, base, tid, nt);
+        }
     }
     // gradient partials are stored grad-major ([i][tile]) so the sum over
     // tiles of one gradient reads one contiguous segment

@@ -1065,7 +1065,7 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
 // skipped.
 void fold_leading_permutations(const Program& prog, PhaseInfo& pi)
 {
-    for (int s = 0; s < kRegBits; s++) pi.gcol[s] = 1u << pi.r[s];
+    for (int s = 0; s < kRegBits; s++) pi.gcol[s] = pi.scol[s];
     pi.gxor = 0;
     while (pi.op_begin < pi.op_end) {
         const DOpHdr& h = prog.ops[pi.op_begin].h;
@@ -1138,16 +1138,34 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         const SOp* s;
         uint32_t lm;      // tile-local bits the op touches
         bool local;       // every qubit of the op is in the tile
+        int fold;         // tile-map candidate: 1 CX (control fc, target ft), 2 SWAP, 0 neither
+        int fc, ft;       // tile-local bits of a candidate
     };
+    // Tile map of a two-state sweep: the amplitude of tile-local index x sits
+    // at LDS index M(x) = XOR of mcol[b] over the set bits b of x.  A CX or
+    // SWAP on two tile bits that are not both register bits of the phase
+    // taking it is not run: after the phase, M becomes M o P (CX: mcol[c] ^=
+    // mcol[t]; SWAP: two columns exchanged), and later phases and the tile's
+    // store address LDS through M.
+    const bool fold_map = prog.n_states == 2 && opt.tile_map;
+    uint32_t mcol[32];
+    for (int l = 0; l < k; l++) mcol[l] = 1u << l;
     std::vector<Pending> remaining;
     remaining.reserve(taken.size());
     uint64_t dep_all = 0;
     for (const SOp* s : taken) {
-        Pending p{s, 0u, true};
+        Pending p{s, 0u, true, 0, -1, -1};
         for (int i = 0; i < s->op->k; i++) {
             const int l = tloc[s->op->q[i]];
             if (l >= 0) p.lm |= 1u << l;
             else p.local = false;
+        }
+        if (fold_map && p.local && !s->grad && !s->diag && s->op->k == 2 && s->op->kind == LOp::DENSE) {
+            const int cx01[4] = {0, 3, 2, 1}, cx10[4] = {0, 1, 3, 2}, sw4[4] = {0, 2, 1, 3};
+            const int l0 = tloc[s->op->q[0]], l1 = tloc[s->op->q[1]];
+            if (is_perm(*s->op, cx01)) p.fold = 1, p.fc = l0, p.ft = l1;
+            else if (is_perm(*s->op, cx10)) p.fold = 1, p.fc = l1, p.ft = l0;
+            else if (is_perm(*s->op, sw4)) p.fold = 2, p.fc = l0, p.ft = l1;
         }
         dep_all |= s->dep;
         remaining.push_back(p);
@@ -1155,20 +1173,32 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     uint32_t n_grad_sweep = 0;
     // ops a phase with fixed register bits R takes; returns a score.  Without
     // output lists the scan stops once every qubit is blocked (nothing later
-    // can be taken or change the score).
-    auto take_with = [&](uint32_t R, std::vector<const SOp*>* ph, std::vector<Pending>* rest) {
-        uint64_t blocked = 0;
+    // can be taken or change the score).  Tile-map folds taken by the phase
+    // (folds) act after it: later ops on their qubits wait for the next
+    // phase, except further folds, which compose in order.
+    auto take_with = [&](uint32_t R, std::vector<const SOp*>* ph, std::vector<Pending>* rest,
+                         std::vector<Pending>* folds) {
+        uint64_t blocked = 0, fblocked = 0;
         long score = 0;
         for (const Pending& p : remaining) {
             const SOp* s = p.s;
-            if (s->dep & blocked) {
+            if ((s->dep & blocked) || (!p.fold && (s->dep & fblocked))) {
                 blocked |= s->dep;
                 if (rest) rest->push_back(p);
                 else if (blocked == dep_all) break;
                 continue;
             }
             const uint32_t lm = p.lm;
-            if (s->diag) {
+            if (p.fold && ((lm & ~R) != 0 || (s->dep & fblocked))) {
+                if (folds) folds->push_back(p);
+                score += 8 * (4L << s->op->k);
+                fblocked |= s->dep;
+                continue;
+            }
+            // (with a tile map, a diagonal with a gradient waits until its tile
+            // qubits are register bits: on group bits its gradient op costs a
+            // full reduction per op where register tables share one)
+            if (s->diag && !(fold_map && s->grad && (lm & ~R) != 0)) {
                 if (ph) ph->push_back(s);
                 if (p.local && (lm & ~R) == 0) score += 1;               // merges into a run
                 else if (lm & R) score -= 1;                             // splits a run
@@ -1191,7 +1221,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         for (const Pending& p : remaining) dep_all |= p.s->dep;
         uint32_t R = 0;  // local-bit mask
         for (const Pending& p : remaining)
-            if (!p.s->diag) {
+            if (!p.s->diag && !p.fold) {
                 R = p.lm;
                 break;
             }
@@ -1203,7 +1233,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             int best_b = -1;
             for (int b = 0; b < k; b++) {
                 if (((R >> b) & 1) || !((cand >> b) & 1)) continue;
-                const long sc = take_with(R | (1u << b), nullptr, nullptr);
+                const long sc = take_with(R | (1u << b), nullptr, nullptr, nullptr);
                 if (sc > best) {
                     best = sc;
                     best_b = b;
@@ -1213,8 +1243,19 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             R |= 1u << best_b;
         }
         std::vector<const SOp*> ph;
-        std::vector<Pending> rest;
-        take_with(R, &ph, &rest);
+        std::vector<Pending> rest, folds;
+        take_with(R, &ph, &rest, &folds);
+        auto apply_folds = [&]() {
+            for (const Pending& f : folds) {
+                if (f.fold == 1) mcol[f.fc] ^= mcol[f.ft];
+                else std::swap(mcol[f.fc], mcol[f.ft]);
+            }
+        };
+        if (ph.empty() && !folds.empty()) {   // only relabelling: no phase
+            apply_folds();
+            remaining.swap(rest);
+            continue;
+        }
         // pad R to rb bits with the lowest free tile bits above the low qubits first
         for (int l = 0; l < k && popc(R) < rb; l++)
             if (!((R >> l) & 1) && l >= c) R |= 1u << l;
@@ -1235,6 +1276,18 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         if (rb < kRegBits) {   // two states: the last register bit selects lambda
             pi.r[kRegBits - 1] = uint32_t(k);
             phys_of_slot[kRegBits - 1] = -1;
+        }
+        // the tile map at this phase: its register and group columns
+        for (int sl = 0; sl < rb; sl++) pi.scol[sl] = mcol[pi.r[sl]];
+        if (rb < kRegBits) pi.scol[kRegBits - 1] = 1u << k;
+        {
+            int gi = 0;
+            for (int l = 0; l < k; l++)
+                if (!((R >> l) & 1)) {
+                    if (gi >= 10) throw std::logic_error("tile map: more than 10 group bits");
+                    pi.gm[gi++] = uint16_t(mcol[l]);
+                }
+            for (int l = 0; l < k; l++) pi.mapped |= mcol[l] != (1u << l) ? 1u : 0u;
         }
         std::vector<const SOp*> merged;
         {
@@ -1296,7 +1349,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                 }
                 merged.clear();
                 for (int q = 0; q < kRegBits; q++)
-                    if ((flip >> q) & 1u) pi.sxor |= 1u << pi.r[q];
+                    if ((flip >> q) & 1u) pi.sxor ^= pi.scol[q];
             }
         }
         for (size_t i = 0; i < merged.size();) {
@@ -1320,9 +1373,16 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         pi.op_end = uint32_t(prog.ops.size());
         fold_leading_permutations(prog, pi);
         prog.phases.push_back(pi);
+        apply_folds();
         remaining.swap(rest);
     }
     sw.phase_end = uint32_t(prog.phases.size());
+    for (int l = 0; l < k; l++) {
+        sw.fcol[l] = uint16_t(mcol[l]);
+        sw.mapped |= mcol[l] != (1u << l) ? 1u : 0u;
+    }
+    if (!sw.mapped)
+        for (int l = 0; l < k; l++) sw.fcol[l] = 0;
     // qubits that select register-table variants become the tile id's high
     // bits: workgroups running at the same time then read the same variants
     for (size_t o = sw.op_begin; o < prog.ops.size(); o++)
@@ -1636,8 +1696,9 @@ std::string describe_program(const Program& prog)
                 const PhaseInfo& ph = prog.phases[p];
                 ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3];
                 bool plain_gather = ph.gxor == 0;
-                for (int b = 0; b < kRegBits; b++) plain_gather = plain_gather && ph.gcol[b] == (1u << ph.r[b]);
+                for (int b = 0; b < kRegBits; b++) plain_gather = plain_gather && ph.gcol[b] == ph.scol[b];
                 if (!plain_gather) ss << " [g]";   // leading permutations folded into the gather
+                if (ph.mapped) ss << " [m]";       // LDS addressed through a tile map
                 if (ph.sxor) ss << " [q]";         // quarter turns: scatter map r ^ f
                 ss << " :";
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {

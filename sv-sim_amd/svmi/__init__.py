@@ -43,7 +43,7 @@ STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_QUBIT", 3: "ERR_OP", 4: "ERR_STATE",
           5: "ERR_DEVICE", 6: "ERR_OOM", 7: "ERR_UNSUPPORTED"}
 
 (OPT_FUSION, OPT_TILE_QUBITS, OPT_LOW_QUBITS, OPT_SWEEP_FLOPS, OPT_PROFILE, OPT_PIPELINE, OPT_OP_COST,
- OPT_COEF_BUDGET, OPT_SPECIALIZE, OPT_MERGE_QUBITS, OPT_PREFETCH) = range(11)
+ OPT_COEF_BUDGET, OPT_SPECIALIZE, OPT_MERGE_QUBITS, OPT_PREFETCH, OPT_TILE_MAP) = range(12)
 
 # Every exported entry point (include/svmi.h, include/svmi_debug.h).
 EXPORTS = ["svmi_create", "svmi_destroy", "svmi_num_qubits", "svmi_reset_state", "svmi_set_random_state",

@@ -79,6 +79,54 @@ def test_adjoint_matches_parameter_shift(n, layers, tile):
     assert np.abs(g - O.param_shift_grad(n, gates, slots, P, terms, state=st)).max() < 1e-12
 
 
+def _rotations_and_permutations(n, layers, rng):
+    """Per layer a rotation with a gradient slot on every qubit (RX/RY theta,
+    RZ phi), then n two-qubit gates on random pairs: CX (either orientation),
+    SWAP and CZ."""
+    gates, slots, p = [], [], 0
+    for _ in range(layers):
+        for q in range(n):
+            name = ("RX", "RY", "RZ")[int(rng.integers(3))]
+            ang = float(rng.random() * 6 - 3)
+            if name == "RZ":
+                gates.append((svmi.OP[name], [q], 0.0, ang, 0.0))
+                slots.append((-1, p, -1))
+            else:
+                gates.append((svmi.OP[name], [q], ang, 0.0, 0.0))
+                slots.append((p, -1, -1))
+            p += 1
+        for _ in range(n):
+            a, b = (int(x) for x in rng.choice(n, size=2, replace=False))
+            gates.append((svmi.OP[("CX", "CX", "SWAP", "CZ")[int(rng.integers(4))]], [a, b]))
+            slots.append((-1, -1, -1))
+    return gates, np.array(slots, dtype=np.int32), p
+
+
+@pytest.mark.parametrize("n,tile", [(9, 6), (12, 8), (13, 11)])
+def test_tile_map_adjoint_matches_parameter_shift(n, tile):
+    # CX and SWAP gates on tile qubits outside a phase's registers relabel the
+    # LDS indices of the two-state tile ([m] phases, the store through the
+    # final map); the gradient must still be the parameter-shift one
+    rng = np.random.default_rng(100 + n)
+    gates, slots, P = _rotations_and_permutations(n, 3, rng)
+    terms = W.ising_terms(n, rng)
+    st = W.random_state(n, seed=n)
+    _, desc = svmi.plan_adjoint(n, gates, slots, P, tile_qubits=tile)
+    assert "[m]" in desc
+    E, g = svmi.emulate_adjoint(n, gates, terms, slots, P, st, tile_qubits=tile)
+    assert abs(E - O.expval(n, *O.run(n, gates, st), terms)) < 1e-12
+    assert np.abs(g - O.param_shift_grad(n, gates, slots, P, terms, state=st)).max() < 1e-12
+
+
+def test_tile_map_folds_the_hea_ladder():
+    # the HEA backward program's CX ladder inside a tile costs no phase
+    gates, slots, P, _ = W.hea_ising(n=20, layers=8)
+    _, desc = svmi.plan_adjoint(20, gates, slots, P)
+    assert "[m]" in desc
+    phases = int(desc.split("phases=")[1].split()[0])
+    assert phases <= 70, phases
+
+
 def test_adjoint_all_parametric_gates_vs_fd():
     n = 5
     rng = np.random.default_rng(9)
