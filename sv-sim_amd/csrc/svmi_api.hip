@@ -398,7 +398,8 @@ svmi_status verify_batch(const BatchPlan& bp)
             if (b.kind != L.kind || (sweep && bp.sweeps[b.sweep].k != L.k) || b.init_zero != L.init ||
                 (b.kind == BSTEP_ADJ && bp.sweeps[b.sweep].n_grad > L.max_grad) ||
                 (sweep && bp.sweeps[b.sweep].n_xf > L.max_xf) ||
-                (sweep && bp.sweeps[b.sweep].max_dense > L.max_dense))
+                (sweep && bp.sweeps[b.sweep].max_dense > L.max_dense) ||
+                (b.kind == BSTEP_FWD && bp.sweeps[b.sweep].mapped && !L.mapped))
                 return fail(SVMI_ERR_STATE, "batch plan: mixed launch");
             nt += b.n_tasks;
         }
@@ -419,7 +420,7 @@ svmi_status verify_batch(const BatchPlan& bp)
             if (b.sweep >= bp.sweeps.size()) return fail(SVMI_ERR_STATE, "batch plan: bad sweep index");
             const SweepInfo& sw = bp.sweeps[b.sweep];
             if (sw.n_tiles != b.n_tasks || sw.k > uint32_t(b.kind == BSTEP_ADJ ? kMaxTileQubitsAdj : kMaxTileQubits) ||
-                sw.k < uint32_t(kRegBits) || sw.phase_end > bp.phases.size() || sw.phase_begin >= sw.phase_end ||
+                sw.k < uint32_t(kRegBits) || sw.phase_end > bp.phases.size() || sw.phase_begin > sw.phase_end ||
                 sw.hitab + (1ull << (sw.k - sw.c)) > bp.hitab.size() || sw.n_grad > uint32_t(kMaxGradsPerSweep))
                 return fail(SVMI_ERR_STATE, "batch plan: bad sweep");
             if ((sw.k + uint32_t(__builtin_popcountll(sw.extmask))) != bp.circs[b.circuit].n)
@@ -431,7 +432,7 @@ svmi_status verify_batch(const BatchPlan& bp)
                 return fail(SVMI_ERR_STATE, "batch plan: bad sweep op range");
             if (sw.mapped) {   // the store's tile map: a bijection of the tile
                 uint32_t basis[32] = {};
-                bool ok = b.kind == BSTEP_ADJ;
+                bool ok = true;
                 for (uint32_t i = 0; i < 14 && ok; i++)
                     ok = i < sw.k ? (sw.fcol[i] < (1u << sw.k) && gf2_insert(basis, sw.fcol[i])) : sw.fcol[i] == 0;
                 if (!ok) return fail(SVMI_ERR_STATE, "batch plan: bad tile map");
@@ -453,7 +454,7 @@ svmi_status verify_batch(const BatchPlan& bp)
                     if (state_bit ? ph.r[r] != sw.k : ph.r[r] >= sw.k)
                         return fail(SVMI_ERR_STATE, "batch plan: bad register bit");
                 }
-                if (!phase_map_valid(ph, sw.k, two) || (!two && ph.mapped))
+                if (!phase_map_valid(ph, sw.k, two) || (ph.mapped && !sw.mapped))
                     return fail(SVMI_ERR_STATE, "batch plan: bad gather map");
                 for (uint32_t o = ph.op_begin; o < ph.op_end; o++)
                     if (bp.ops[o].h.coef > bp.coef.size()) return fail(SVMI_ERR_STATE, "batch plan: bad coef offset");
@@ -667,6 +668,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
             const uint32_t ng = b.kind == BSTEP_ADJ ? bp.sweeps[b.sweep].n_grad : 0u;
             const uint32_t nx = (b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ) ? bp.sweeps[b.sweep].n_xf : 0u;
             const uint32_t md = (b.kind == BSTEP_FWD || b.kind == BSTEP_ADJ) ? bp.sweeps[b.sweep].max_dense : 0u;
+            const uint32_t mp = b.kind == BSTEP_FWD ? bp.sweeps[b.sweep].mapped : 0u;
             if (!bp.launches.empty() && bp.launches.back().step_end == idx &&
                 bp.launches.back().step_begin >= bp.level_begin.back() && bp.launches.back().kind == b.kind &&
                 bp.launches.back().k == k && bp.launches.back().init == b.init_zero) {
@@ -676,6 +678,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
                 L.max_grad = std::max(L.max_grad, ng);
                 L.max_xf = std::max(L.max_xf, nx);
                 L.max_dense = std::max(L.max_dense, md);
+                L.mapped |= mp;
             } else {
                 BatchLaunch L;
                 std::memset(&L, 0, sizeof(L));
@@ -686,6 +689,7 @@ svmi_status build_batch(const svmi_batch_item* items, size_t n_items, const doub
                 L.max_grad = ng;
                 L.max_xf = nx;
                 L.max_dense = md;
+                L.mapped = mp;
                 L.init = b.init_zero;
                 L.task_begin = b.task_begin;
                 L.n_tasks = b.n_tasks;
@@ -1088,6 +1092,7 @@ PlanOptions forward_options(svmi_handle h)
 {
     PlanOptions o = h->opt;
     if (h->specialize && !h->op_cost_set) o.op_overhead = 0;
+    if (h->specialize) o.tile_map = false;   // the compiled sweeps address LDS without maps
     return o;
 }
 
@@ -1714,6 +1719,9 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
         t[l] = init_zero ? c64{gidx[l] == 0 ? 1.0 : 0.0, 0.0} : s0[gidx[l]];
         if (ns == 2) t[tsize + l] = s1[gidx[l]];
     }
+    // two-state sweeps always address LDS through their tile maps (the
+    // identity where nothing was folded), one-state sweeps when mapped
+    const bool mp = ns == 2 || sw.mapped;
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
         const PhaseInfo& ph = phases[p];
         uint32_t off[kRegs], goffs[kRegs];
@@ -1723,7 +1731,7 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
         phase_scatter_offsets(ph, soffs);
         for (uint32_t g = 0; g < ngroups; g++) {
             const uint32_t goff = group_offset(ph, g);
-            const uint32_t gphys = ns == 2 ? mapped_group_offset(ph, g) : goff;
+            const uint32_t gphys = mp ? mapped_group_offset(ph, g) : goff;
             c64 a[kRegs];
             for (int r = 0; r < kRegs; r++) a[r] = t[gphys ^ goffs[r]];
             for (uint32_t o = ph.op_begin; o < ph.op_end; o++) {
@@ -1734,11 +1742,11 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
                     exec_state_op(op.h, &op, a, coef, goff, g, off, base, xf.data(), sw.exthi,
                                   uint32_t(extract_bits(base, sw.exthi)));
             }
-            for (int r = 0; r < kRegs; r++) t[ns == 2 ? (gphys ^ soffs[r]) : ((goff | off[r]) ^ ph.sxor)] = a[r];
+            for (int r = 0; r < kRegs; r++) t[mp ? (gphys ^ soffs[r]) : ((goff | off[r]) ^ ph.sxor)] = a[r];
         }
     }
     for (uint32_t l = 0; l < tsize; l++) {
-        const uint32_t m = ns == 2 && sw.mapped ? sweep_map(sw, l) : l;
+        const uint32_t m = sw.mapped ? sweep_map(sw, l) : l;
         s0[gidx[l]] = t[m];
         if (ns == 2) s1[gidx[l]] = t[tsize + m];
     }
@@ -1988,7 +1996,9 @@ svmi_status svmi_debug_jit(int n, const svmi_gate* gates, size_t n_gates, int ti
     if (tile_qubits < kRegBits || tile_qubits > kMaxTileQubits) return fail(SVMI_ERR_ARG, "bad tile_qubits");
     SVMI_TRY(check_gates(gates, n_gates, n));
     Program prog;
-    plan_forward(gates, n_gates, std::max(n, kRegBits), debug_options(1, tile_qubits, low_qubits, 0), prog);
+    PlanOptions o = debug_options(1, tile_qubits, low_qubits, 0);
+    o.tile_map = false;   // as SVMI_OPT_SPECIALIZE plans
+    plan_forward(gates, n_gates, std::max(n, kRegBits), o, prog);
     std::vector<char> covered;
     const uint64_t dma_grid = compile == 2 ? 256 : 0;   // double-buffered kernels for a 256-CU device
     const std::string src = jit_source(prog, covered, dma_grid);

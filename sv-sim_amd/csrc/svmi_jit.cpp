@@ -339,7 +339,7 @@ std::string jit_source(const Program& prog, std::vector<char>& covered, uint64_t
     if (prog.n_states != 1) return o.str();
     for (uint32_t i = 0; i < prog.sweeps.size(); i++) {
         const SweepInfo& sw = prog.sweeps[i];
-        if (sw.k > kJitMaxTileQubits) continue;
+        if (sw.k > kJitMaxTileQubits || sw.mapped) continue;   // tile maps: interpreted
         if (dma_grid && dma_eligible(sw) && sw.n_tiles >= 2 * dma_grid) {
             emit_sweep_dma(o, prog, i);
             covered[i] = 2;

@@ -108,7 +108,9 @@ __device__ __attribute__((always_inline)) inline double wave_sum_lane63(double v
 // kernel: blockDim >= nt) the others only join the barriers.  Without kIdle
 // every thread is active, which keeps the phase body free of divergence
 // (and its register groups out of scratch).  Ends with a barrier.
-template <int NS, bool kIdle, int kMaxQ>
+// kMap: the phases address LDS through their tile maps (every two-state
+// sweep; one-state sweeps whose SweepInfo::mapped is set).
+template <int NS, bool kIdle, int kMaxQ, bool kMap = (NS == 2)>
 __device__ __attribute__((always_inline)) inline void
 tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
             const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops, const c64* __restrict__ coef,
@@ -130,10 +132,10 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
             // at LDS indices under the tile map (gphys, gather and scatter
             // offsets); the ops see the plain tile-local indices (goff, off).
             uint32_t gphys = goff;
-            if constexpr (NS == 2) gphys = mapped_group_offset(ph, tid);
+            if constexpr (kMap) gphys = mapped_group_offset(ph, tid);
             c64 a[kRegs];
 #pragma unroll
-            for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(NS == 2 ? (gphys ^ goffs[r]) : (goff | goffs[r]))];
+            for (int r = 0; r < kRegs; r++) a[r] = t0[lds_slot(kMap ? (gphys ^ goffs[r]) : (goff | goffs[r]))];
             // op headers are fetched one op ahead so the scalar-load latency
             // of the next op overlaps the arithmetic of the current one (every
             // op array ends with a NOP, so the read past a phase's last op
@@ -161,7 +163,7 @@ tile_phases(c64* t0, double* gacc, const c64* xf, const SweepInfo& sw,
                 }
                 exec_state_op<kMaxQ>(h, op, a, coef, goff, tid, off, base, xf, sw.exthi, vhi);
             }
-            if constexpr (NS == 2) {
+            if constexpr (kMap) {
                 uint32_t soffs[kRegs];
                 phase_scatter_offsets(ph, soffs);
 #pragma unroll
@@ -196,7 +198,7 @@ tile_factors(c64* xf, const SweepInfo& sw, const DOp* __restrict__ ops, const c6
 // gradient accumulators.
 // One tile of one sweep: load, phases, store (and for NS = 2 the tile's
 // gradient partials).  init_zero: the tile is generated as |0..0>.
-template <int NS, bool kIdle, int kMaxQ = (NS == 2 ? kMaxDenseQubitsAdj : kRegBits)>
+template <int NS, bool kIdle, int kMaxQ = (NS == 2 ? kMaxDenseQubitsAdj : kRegBits), bool kMap = (NS == 2)>
 __device__ __attribute__((always_inline)) inline void
 sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
            const PhaseInfo* __restrict__ phases, const DOp* __restrict__ ops,
@@ -227,11 +229,11 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
         for (uint32_t i = tid; i < sw.n_grad * nwaves; i += blockDim.x) gacc[i] = 0.0;
     tile_factors(xf, sw, ops, coef, base, tid);
     __syncthreads();
-    tile_phases<NS, kIdle, kMaxQ>(t0, gacc, xf, sw, phases, ops, coef, base, tid, nt);
+    tile_phases<NS, kIdle, kMaxQ, kMap>(t0, gacc, xf, sw, phases, ops, coef, base, tid, nt);
     if (active) {
-        if (NS == 2 && sw.mapped) {
+        if (kMap && sw.mapped) {
             tile_store_mapped<J>(t0, 0u, s0, sw, ht, base, tid, nt);
-            tile_store_mapped<J>(t0, tsize, s1, sw, ht, base, tid, nt);
+            if constexpr (NS == 2) tile_store_mapped<J>(t0, tsize, s1, sw, ht, base, tid, nt);
         } else {
             tile_store<J>(t0, 0u, s0, sw, ht, base, tid, nt);
             if constexpr (NS == 2) tile_store<J>(t0, tsize, s1, sw, ht, base, tid, nt);
@@ -255,7 +257,8 @@ sweep_tile(c64* __restrict__ s0, c64* __restrict__ s1, const SweepInfo& sw,
 // workgroups of a CU alternate loading and computing.
 // kMaxQ: widest dense block compiled in (sweeps with blocks of at most two
 // register qubits run the kMaxQ = 2 build, which holds fewer registers).
-template <int kMaxQ>
+// kMap: the build for sweeps with tile maps (SweepInfo::mapped).
+template <int kMaxQ, bool kMap>
 __global__ void __launch_bounds__(kMaxTileThreads, 1)
 k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__ phases,
          const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab,
@@ -264,7 +267,7 @@ k_sweep1(c64* __restrict__ s0, const SweepInfo sw, const PhaseInfo* __restrict__
     extern __shared__ __align__(16) unsigned char smem[];
     if (blockIdx.x >= stagger_lo && blockIdx.x < stagger_hi)
         for (uint32_t i = 0; i < stagger; i++) __builtin_amdgcn_s_sleep(127);
-    sweep_tile<1, false, kMaxQ>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x,
+    sweep_tile<1, false, kMaxQ, kMap>(s0, nullptr, sw, phases, ops, coef, hitab, nullptr, tile_offset + blockIdx.x, blockDim.x,
                          smem, false);
 }
 
@@ -411,7 +414,8 @@ __device__ inline uint32_t batch_step_of(const BatchStep* __restrict__ steps, ui
         const DOp* __restrict__ ops, const c64* __restrict__ coef, const uint64_t* __restrict__ hitab
 
 // kInit: the steps are first forward sweeps (tiles generated as |0..0>).
-template <bool kInit, int kMaxQ>
+// kMap: some of the launch's sweeps have tile maps.
+template <bool kInit, int kMaxQ, bool kMap>
 __global__ void __launch_bounds__(kMaxTileThreads, 1)
 k_batch_fwd(SVMI_BATCH_TABLES, uint32_t s0, uint32_t s1, uint64_t task0)
 {
@@ -419,7 +423,7 @@ k_batch_fwd(SVMI_BATCH_TABLES, uint32_t s0, uint32_t s1, uint64_t task0)
     const uint64_t t = task0 + blockIdx.x;
     const BatchStep st = steps[batch_step_of(steps, s0, s1, t)];
     const SweepInfo sw = sweeps[st.sweep];
-    sweep_tile<1, false, kMaxQ>(circuits[st.circuit].phi, nullptr, sw, phases, ops, coef, hitab, nullptr,
+    sweep_tile<1, false, kMaxQ, kMap>(circuits[st.circuit].phi, nullptr, sw, phases, ops, coef, hitab, nullptr,
                          t - st.task_begin, blockDim.x, smem, kInit);
 }
 
@@ -506,6 +510,20 @@ size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad, uint32_t 
     return 0;
 }
 
+// The k_batch_fwd build of a forward launch.
+using BatchFwdFn = void (*)(const BatchStep*, const BatchCircuit*, const SweepInfo*, const PhaseInfo*, const DOp*,
+                            const c64*, const uint64_t*, uint32_t, uint32_t, uint64_t);
+template <bool kInit, bool kMap>
+BatchFwdFn batch_fwd_kernel_of(const BatchLaunch& L)
+{
+    return L.max_dense <= 2 ? &k_batch_fwd<kInit, 2, kMap> : &k_batch_fwd<kInit, kRegBits, kMap>;
+}
+BatchFwdFn batch_fwd_kernel(const BatchLaunch& L)
+{
+    if (L.init) return L.mapped ? batch_fwd_kernel_of<true, true>(L) : batch_fwd_kernel_of<true, false>(L);
+    return L.mapped ? batch_fwd_kernel_of<false, true>(L) : batch_fwd_kernel_of<false, false>(L);
+}
+
 hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L)
 {
     const size_t smem = batch_launch_smem(L.kind, L.k, L.max_grad, L.max_xf);
@@ -519,8 +537,7 @@ hipError_t launch_batch(hipStream_t st, const BatchArgs& a, const BatchLaunch& L
         const uint64_t first = L.task_begin + t0;
         switch (L.kind) {
         case BSTEP_FWD: {
-            auto* fn = L.init ? (L.max_dense <= 2 ? k_batch_fwd<true, 2> : k_batch_fwd<true, kRegBits>)
-                              : (L.max_dense <= 2 ? k_batch_fwd<false, 2> : k_batch_fwd<false, kRegBits>);
+            auto* fn = batch_fwd_kernel(L);
             hipLaunchKernelGGL(fn, dim3(cnt), dim3(threads), smem, st, a.steps, a.circuits, a.sweeps, a.phases,
                                a.ops, a.coef, a.hitab, L.step_begin, L.step_end, first);
             break;
@@ -589,6 +606,8 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
     const uint32_t nt = sweep_threads(sw.k, ns);
     const size_t smem = sweep_smem_bytes(sw, ns);
     if (smem > g_max_smem) return hipErrorInvalidConfiguration;
+    // sweeps with tile maps run on k_sweep1's mapped build only
+    if (ns == 1 && sw.mapped) pipeline = 0, prefetch = false;
     if (ns == 1 && prefetch && pipeline == 0 && sw.k == kDmaTileQubits && nt == kDmaThreads &&
         dma_sweep_smem_bytes(sw) <= g_max_smem) {
         const uint64_t grid = dma_grid();
@@ -623,8 +642,10 @@ hipError_t launch_sweep(hipStream_t st, int ns, c64* s0, c64* s1, const SweepInf
                 hi = uint32_t(res);
                 sl = uint32_t(pipeline - 2);
             }
-            hipLaunchKernelGGL(sw.max_dense <= 2 ? k_sweep1<2> : k_sweep1<kRegBits>, dim3(uint32_t(cnt)), dim3(nt),
-                               smem, st, s0, sw, phases, ops, coef, hitab, t0, lo, hi, sl);
+            auto* fn = sw.mapped ? (sw.max_dense <= 2 ? &k_sweep1<2, true> : &k_sweep1<kRegBits, true>)
+                                 : (sw.max_dense <= 2 ? &k_sweep1<2, false> : &k_sweep1<kRegBits, false>);
+            hipLaunchKernelGGL(fn, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, sw, phases, ops, coef, hitab, t0, lo,
+                               hi, sl);
         }
         else
             hipLaunchKernelGGL(k_sweep2, dim3(uint32_t(cnt)), dim3(nt), smem, st, s0, s1, sw, phases, ops,
@@ -648,10 +669,16 @@ hipError_t init_sweep_kernels()
     size_t want = size_t(optin);
     if (want > 160 * 1024) want = 160 * 1024;
     if (want > 64 * 1024) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<kRegBits>),
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<kRegBits, false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<2>),
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<2, false>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<kRegBits, true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1<2, true>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(want));
         if (e == hipSuccess)
             e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep1p<false>),
@@ -673,17 +700,19 @@ hipError_t init_sweep_kernels()
             want = 64 * 1024;
         }
     }
+    auto raise = [&](const void* fn) {
+        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) == hipSuccess;
+    };
     if (want > 64 * 1024 &&
-        (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<true, kRegBits>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<false, kRegBits>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<true, 2>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_fwd<false, 2>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(want)) != hipSuccess ||
-         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_adj), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             int(want)) != hipSuccess)) {
+        !(raise(reinterpret_cast<const void*>(&k_batch_fwd<true, kRegBits, false>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_fwd<false, kRegBits, false>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_fwd<true, 2, false>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_fwd<false, 2, false>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_fwd<true, kRegBits, true>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_fwd<false, kRegBits, true>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_fwd<true, 2, true>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_fwd<false, 2, true>)) &&
+          raise(reinterpret_cast<const void*>(&k_batch_adj)))) {
         (void)hipGetLastError();
         want = 64 * 1024;
     }

@@ -68,6 +68,7 @@ struct BatchLaunch {
     uint32_t init;          // FWD: first sweeps (tiles start as |0..0>)
     uint32_t max_xf;        // largest n_xf of the group's sweeps
     uint32_t max_dense;     // FWD: widest dense block of the group's sweeps
+    uint32_t mapped;        // FWD: some of the group's sweeps have tile maps
     uint64_t task_begin, n_tasks;
 };
 size_t batch_launch_smem(uint32_t kind, uint32_t k, uint32_t max_grad, uint32_t max_xf);

@@ -1092,6 +1092,10 @@ void emit_standalone(Program& prog, const SOp& s)
     prog.standalone.push_back(so);
 }
 
+// One-state sweeps of at most 2^kMapMaxTileBits1 tiles (two rounds of
+// resident workgroups on a 256-CU device) use tile maps.
+constexpr int kMapMaxTileBits1 = 10;
+
 void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T, int n, int c,
                 const PlanOptions& opt)
 {
@@ -1141,13 +1145,15 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         int fold;         // tile-map candidate: 1 CX (control fc, target ft), 2 SWAP, 0 neither
         int fc, ft;       // tile-local bits of a candidate
     };
-    // Tile map of a two-state sweep: the amplitude of tile-local index x sits
-    // at LDS index M(x) = XOR of mcol[b] over the set bits b of x.  A CX or
-    // SWAP on two tile bits that are not both register bits of the phase
-    // taking it is not run: after the phase, M becomes M o P (CX: mcol[c] ^=
-    // mcol[t]; SWAP: two columns exchanged), and later phases and the tile's
-    // store address LDS through M.
-    const bool fold_map = prog.n_states == 2 && opt.tile_map;
+    // Tile map of a sweep: the amplitude of tile-local index x sits at LDS
+    // index M(x) = XOR of mcol[b] over the set bits b of x.  A CX or SWAP on
+    // two tile bits that are not both register bits of the phase taking it is
+    // not run: after the phase, M becomes M o P (CX: mcol[c] ^= mcol[t]; SWAP:
+    // two columns exchanged), and later phases and the tile's store address
+    // LDS through M.  One-state sweeps over many tiles stream at HBM speed,
+    // where the phases saved buy nothing and the mapped addresses cost LDS
+    // bank conflicts: those keep the plain layout.
+    const bool fold_map = opt.tile_map && (prog.n_states == 2 || (n - k) <= kMapMaxTileBits1);
     uint32_t mcol[32];
     for (int l = 0; l < k; l++) mcol[l] = 1u << l;
     std::vector<Pending> remaining;
@@ -1371,18 +1377,29 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
             enc.encode(*s, tloc, slot_of, gslot);
         }
         pi.op_end = uint32_t(prog.ops.size());
+        const uint32_t ops0 = pi.op_begin;
         fold_leading_permutations(prog, pi);
+        if (fold_map && pi.op_begin == pi.op_end && pi.gxor == 0 && pi.sxor == 0) {
+            // the phase only permutes its registers (linearly): instead of
+            // moving the amplitudes, register bit r[s] of the tile map takes
+            // the gather column the phase would have read it from
+            for (int sl = 0; sl < rb; sl++) mcol[pi.r[sl]] = pi.gcol[sl];
+            prog.ops.resize(ops0);
+            apply_folds();
+            remaining.swap(rest);
+            continue;
+        }
         prog.phases.push_back(pi);
         apply_folds();
         remaining.swap(rest);
     }
     sw.phase_end = uint32_t(prog.phases.size());
-    for (int l = 0; l < k; l++) {
-        sw.fcol[l] = uint16_t(mcol[l]);
-        sw.mapped |= mcol[l] != (1u << l) ? 1u : 0u;
-    }
-    if (!sw.mapped)
-        for (int l = 0; l < k; l++) sw.fcol[l] = 0;
+    // mapped: some phase or the store addresses LDS through a map (fcol = the
+    // final one, which may be the identity)
+    for (int l = 0; l < k; l++) sw.mapped |= mcol[l] != (1u << l) ? 1u : 0u;
+    for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) sw.mapped |= prog.phases[p].mapped;
+    if (sw.mapped)
+        for (int l = 0; l < k; l++) sw.fcol[l] = uint16_t(mcol[l]);
     // qubits that select register-table variants become the tile id's high
     // bits: workgroups running at the same time then read the same variants
     for (size_t o = sw.op_begin; o < prog.ops.size(); o++)
@@ -1426,7 +1443,9 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         return w;
     };
     sw.max_dense = dense_width(sw.op_begin, sw.op_end);
-    if (prog.n_states == 1 && opt.coef_budget > 0 && sw.n_tiles >= 512) {
+    // (a sweep with a tile map stays whole: its parts would reload the tile
+    // in the plain layout)
+    if (prog.n_states == 1 && opt.coef_budget > 0 && sw.n_tiles >= 512 && !sw.mapped) {
         // bytes a tile reads through the scalar cache per phase: every op
         // record (DOp stride) and the coefficients the op loads
         auto phase_bytes = [&](const PhaseInfo& ph) {
@@ -1691,7 +1710,9 @@ std::string describe_program(const Program& prog)
                     ss << (first ? "" : ",") << p;
                     first = false;
                 }
-            ss << "]\n";
+            ss << "]";
+            if (sw.mapped) ss << " [m]";   // phases or the store address LDS through a tile map
+            ss << "\n";
             for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) {
                 const PhaseInfo& ph = prog.phases[p];
                 ss << "    R=" << ph.r[0] << "," << ph.r[1] << "," << ph.r[2] << "," << ph.r[3];

@@ -20,8 +20,8 @@ struct PlanOptions {
     bool rotations = true;     // one-state phases: one-qubit unitaries as lifting rotations between
                                // merged register tables where the cost model prefers it
     int merge_qubits = 2;      // widest dense block the register-run merging may form (1..4)
-    bool tile_map = true;      // two-state sweeps: tile-local CX and SWAP ops outside a phase's
-                               // register bits become a relabelling of LDS indices, not a phase
+    bool tile_map = true;      // tile-local CX and SWAP ops outside a phase's register bits become
+                               // a relabelling of the tile's LDS indices, not a phase
 };
 
 // One step of a program: a tiled sweep or a standalone full-state op.

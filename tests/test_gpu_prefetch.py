@@ -33,6 +33,9 @@ def _random_state(n, seed):
 
 def _run(n, gates, prefetch, state=None):
     with svmi.Engine(n) as e:
+        # without tile maps: sweeps with maps run on the one-tile-per-workgroup
+        # kernel whatever the prefetch setting
+        e.set_option(svmi.OPT_TILE_MAP, 0)
         e.set_option(svmi.OPT_PREFETCH, prefetch)
         e.upload(gates)
         if state is not None:
@@ -68,6 +71,7 @@ def test_prefetch_runs_twice_on_one_handle():
     g1 = W.u3_brickwall(n=n, layers=3)
     g2 = W.random_layers(n=n, layers=6)
     with svmi.Engine(n) as e:
+        e.set_option(svmi.OPT_TILE_MAP, 0)
         e.set_option(svmi.OPT_PREFETCH, 1)
         e.upload(g1)
         e.run()

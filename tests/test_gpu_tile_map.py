@@ -1,11 +1,13 @@
-"""GPU parity of the tile map of adjoint sweeps (SVMI_OPT_TILE_MAP).
+"""GPU parity of the tile map of sweeps (SVMI_OPT_TILE_MAP).
 
 With the map on (the default), CX and SWAP gates on two tile qubits that a
 phase does not hold in registers are not run: they relabel the LDS indices
-of the two-state tile, later phases gather and scatter through the map and
-the tile's store writes through the final one.  The gradients and <H> must
-agree with the map off (a different phase schedule, so to rounding, 1e-12)
-and with the oracle's parameter-shift gradient.
+of the tile, later phases gather and scatter through the map and the tile's
+store writes through the final one.  Adjoint (two-state) sweeps: gradients
+and <H> agree with the map off (a different phase schedule, so to rounding,
+1e-12) and with the oracle's parameter-shift gradient.  Forward (one-state)
+sweeps, which run on k_sweep1's mapped build: states agree with the map off
+and with the oracle (1e-10).
 """
 import numpy as np
 import pytest
@@ -69,3 +71,34 @@ def test_permutation_mix_matches_parameter_shift(n):
     gp = O.param_shift_grad(n, gates, slots, P, terms, state=st)
     assert np.abs(g1 - gp).max() < 1e-11
     assert np.abs(g1 - g0).max() < 1e-12
+
+
+def _state(n, gates, tile_map, state=None, tile=None):
+    with svmi.Engine(n) as e:
+        e.set_option(svmi.OPT_TILE_MAP, tile_map)
+        if tile:
+            e.set_option(svmi.OPT_TILE_QUBITS, tile)
+        e.upload(gates)
+        if state is not None:
+            e.set_state(*state)
+        e.run()
+        return e.get_state()
+
+
+@pytest.mark.parametrize("n,name,tile", [(12, "random_layers", None), (20, "random_layers", None),
+                                         (21, "random_layers", 13), (21, "qft", None), (18, "qft", 8)])
+def test_forward_tile_map_matches_oracle(n, name, tile):
+    state = None
+    if name == "random_layers":
+        gates = W.random_layers(n=n, layers=12)
+    else:
+        gates = W.qft(n)
+        state = W.random_state(n, seed=n)
+    _, desc = svmi.plan(n, gates, tile_qubits=tile or 12)
+    assert "[m]" in desc
+    print("case", n, name, tile, len(gates), "gates", flush=True)
+    re1, im1 = _state(n, gates, 1, state, tile)
+    re0, im0 = _state(n, gates, 0, state, tile)
+    ore, oim = O.run(n, gates, state)
+    assert max(np.abs(re1 - ore).max(), np.abs(im1 - oim).max()) < 1e-10
+    assert max(np.abs(re1 - re0).max(), np.abs(im1 - im0).max()) < 1e-12

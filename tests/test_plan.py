@@ -118,6 +118,20 @@ def test_tile_map_adjoint_matches_parameter_shift(n, tile):
     assert np.abs(g - O.param_shift_grad(n, gates, slots, P, terms, state=st)).max() < 1e-12
 
 
+@pytest.mark.parametrize("n,tile,low", [(12, 12, 4), (14, 8, 2), (16, 6, 2), (17, 12, 4)])
+def test_one_state_tile_map_matches_oracle(n, tile, low):
+    # forward sweeps fold CX and SWAP gates into the tile map as well; the
+    # store writes the tile back through the final map
+    gates = W.random_layers(n=n, layers=8)
+    rng = np.random.default_rng(n)
+    swaps = [(svmi.OP["SWAP"], [int(a), int(b)]) for a, b in rng.integers(0, n, size=(6, 2)) if a != b]
+    gates = np.concatenate([svmi.as_gates(gates), svmi.as_gates(swaps)])
+    _, desc = svmi.plan(n, gates, tile_qubits=tile, low_qubits=low)
+    assert "[m]" in desc
+    st = W.random_state(n, seed=n + 7)
+    assert _err(svmi.emulate(n, gates, st, tile_qubits=tile, low_qubits=low), O.run(n, gates, st)) < 1e-12
+
+
 def test_tile_map_folds_the_hea_ladder():
     # the HEA backward program's CX ladder inside a tile costs no phase
     gates, slots, P, _ = W.hea_ising(n=20, layers=8)
