@@ -1092,8 +1092,8 @@ void emit_standalone(Program& prog, const SOp& s)
     prog.standalone.push_back(so);
 }
 
-// One-state sweeps of at most 2^kMapMaxTileBits1 tiles (two rounds of
-// resident workgroups on a 256-CU device) use tile maps.
+// One-state sweeps of at most 2^kMapMaxTileBits1 tiles (twice the
+// workgroups a 256-CU device holds at once) use tile maps.
 constexpr int kMapMaxTileBits1 = 10;
 
 void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T, int n, int c,
