@@ -435,6 +435,9 @@ svmi_status verify_batch(const BatchPlan& bp)
                 bool ok = true;
                 for (uint32_t i = 0; i < 14 && ok; i++)
                     ok = i < sw.k ? (sw.fcol[i] < (1u << sw.k) && gf2_insert(basis, sw.fcol[i])) : sw.fcol[i] == 0;
+                const uint32_t rb = uint32_t(b.kind == BSTEP_ADJ ? kRegBitsAdj : kRegBits);
+                for (uint32_t j = 0; j < 16 && ok; j++)
+                    ok = j < (1u << rb) ? sw.fj[j] == sweep_map(sw, j << (sw.k - rb)) : sw.fj[j] == 0;
                 if (!ok) return fail(SVMI_ERR_STATE, "batch plan: bad tile map");
             }
             for (uint32_t o = sw.op_begin; o < sw.op_end; o++)
@@ -1745,8 +1748,10 @@ void host_tile(const SweepInfo& sw, const PhaseInfo* phases, const DOp* ops, con
             for (int r = 0; r < kRegs; r++) t[mp ? (gphys ^ soffs[r]) : ((goff | off[r]) ^ ph.sxor)] = a[r];
         }
     }
+    const uint32_t tbits = sw.k - uint32_t(ns == 2 ? kRegBitsAdj : kRegBits);   // log2 of the tile's threads
     for (uint32_t l = 0; l < tsize; l++) {
-        const uint32_t m = sw.mapped ? sweep_map(sw, l) : l;
+        // as the kernels' store: M(tid) ^ M(nt j) with l = tid + nt j
+        const uint32_t m = sw.mapped ? sweep_map_thread(sw, l & ((1u << tbits) - 1u)) ^ sw.fj[l >> tbits] : l;
         s0[gidx[l]] = t[m];
         if (ns == 2) s1[gidx[l]] = t[tsize + m];
     }

@@ -486,6 +486,17 @@ SVMI_HD inline uint32_t sweep_map(const SweepInfo& sw, uint32_t l)
     return o;
 }
 
+// The thread part of the store's map: M(tid) for tid below the tile's
+// thread count (at most 2^10 threads).
+SVMI_HD inline uint32_t sweep_map_thread(const SweepInfo& sw, uint32_t tid)
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < 10; b++)
+        if ((tid >> b) & 1u) o ^= sw.fcol[b];
+    return o;
+}
+
 // Tile-local base index of group g: insert zero bits at the 4 register bits.
 SVMI_HD inline uint32_t group_offset(const PhaseInfo& ph, uint32_t g)
 {

@@ -266,8 +266,9 @@ struct SweepInfo {
                          // with max_dense <= 2 run a kernel compiled without wider blocks
     uint32_t lt_coef;    // coef offset of a contiguous copy of those lane tables (16 c64 each),
     uint32_t n_lt;       // which every tile copies into the end of its factor area
-    uint32_t mapped;     // two-state sweeps: the tile map after the last phase is not the
-    uint16_t fcol[14];   // identity; fcol[b] = M(1 << b), read by the tile's store
+    uint32_t mapped;     // some phase or the tile's store addresses LDS through a tile map M
+    uint16_t fcol[14];   // M after the last phase: fcol[b] = M(1 << b)
+    uint16_t fj[16];     // M(j nt) (nt = the tile's threads): the store's per-register part
 };
 
 // A standalone op (run by its own kernel on the whole state, outside sweeps).

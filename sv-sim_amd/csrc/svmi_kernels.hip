@@ -49,18 +49,19 @@ tile_put(const c64 (&r)[J], c64* t, uint32_t lo, uint32_t tid, uint32_t nt)
 }
 
 // LDS -> HBM for the tile at `base` when the sweep ends with a tile map:
-// tile-local index l = tid + nt j sits at LDS index M(tid) ^ M(nt j).
+// tile-local index l = tid + nt j sits at LDS index M(tid) ^ M(nt j), the
+// second part precomputed per sweep (SweepInfo::fj).
 template <int J>
 __device__ __attribute__((always_inline)) inline void
 tile_store_mapped(const c64* t, uint32_t lo, c64* __restrict__ s, const SweepInfo& sw,
                   const uint64_t* __restrict__ ht, uint64_t base, uint32_t tid, uint32_t nt)
 {
     const uint32_t cmask = (1u << sw.c) - 1u;
-    const uint32_t mt = sweep_map(sw, tid);
+    const uint32_t mt = sweep_map_thread(sw, tid);
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const uint32_t l = tid + nt * j;
-        s[base | ht[l >> sw.c] | (l & cmask)] = t[lds_slot(lo + (mt ^ sweep_map(sw, nt * j)))];
+        s[base | ht[l >> sw.c] | (l & cmask)] = t[lds_slot(lo + (mt ^ sw.fj[j]))];
     }
 }
 

@@ -1063,6 +1063,94 @@ long compile_phase(const std::vector<const SOp*>& ops, const int* tloc, const in
 // so the map g becomes g o P (X on slot s: gxor ^= gcol[s]; CX control c,
 // target t: gcol[c] ^= gcol[t]; SWAP: exchange two columns) and the ops are
 // skipped.
+// The tile map at the start of a phase (col[b] = M(1 << b), b < k), read
+// back from its register and group columns.
+void phase_map_columns(const PhaseInfo& ph, int k, int rb, uint32_t* col)
+{
+    uint32_t R = 0;
+    for (int s = 0; s < rb; s++) {
+        R |= 1u << ph.r[s];
+        col[ph.r[s]] = ph.scol[s];
+    }
+    int gi = 0;
+    for (int l = 0; l < k; l++)
+        if (!((R >> l) & 1u)) col[l] = ph.gm[gi++];
+}
+
+uint32_t map_apply(const uint32_t* col, int k, uint32_t v)
+{
+    uint32_t o = 0;
+    for (int b = 0; b < k; b++)
+        if ((v >> b) & 1u) o ^= col[b];
+    return o;
+}
+
+// Columns of the inverse of a bijective GF(2)-linear map of k bits
+// (Gauss-Jordan on its rows).
+void map_inverse(const uint32_t* col, int k, uint32_t* inv)
+{
+    uint32_t row[32], aug[32];
+    for (int i = 0; i < k; i++) {
+        row[i] = 0;
+        for (int b = 0; b < k; b++) row[i] |= ((col[b] >> i) & 1u) << b;
+        aug[i] = 1u << i;
+    }
+    for (int c = 0; c < k; c++) {
+        int piv = c;
+        while (piv < k && !((row[piv] >> c) & 1u)) piv++;
+        if (piv == k) throw std::logic_error("tile map: not invertible");
+        std::swap(row[c], row[piv]);
+        std::swap(aug[c], aug[piv]);
+        for (int i = 0; i < k; i++)
+            if (i != c && ((row[i] >> c) & 1u)) {
+                row[i] ^= row[c];
+                aug[i] ^= aug[c];
+            }
+    }
+    for (int j = 0; j < k; j++) {
+        inv[j] = 0;
+        for (int i = 0; i < k; i++) inv[j] |= ((aug[i] >> j) & 1u) << i;
+    }
+}
+
+// A sweep cut into parts reloads the tile in the plain layout at every part:
+// the part's phases address LDS through N o M_p, N the inverse of the map at
+// the part's first phase, and its store through N o (map at the part's end).
+void rebase_part(Program& prog, SweepInfo& part, int rb, const uint32_t* end_col)
+{
+    const int k = int(part.k);
+    uint32_t start[32], N[32];
+    phase_map_columns(prog.phases[part.phase_begin], k, rb, start);
+    map_inverse(start, k, N);
+    part.mapped = 0;
+    for (uint32_t p = part.phase_begin; p < part.phase_end; p++) {
+        PhaseInfo& ph = prog.phases[p];
+        for (int s = 0; s < rb; s++) {
+            ph.scol[s] = map_apply(N, k, ph.scol[s]);
+            ph.gcol[s] = map_apply(N, k, ph.gcol[s]);
+        }
+        for (int i = 0; i < k - rb; i++) ph.gm[i] = uint16_t(map_apply(N, k, ph.gm[i]));
+        ph.gxor = map_apply(N, k, ph.gxor);
+        ph.sxor = map_apply(N, k, ph.sxor);
+        uint32_t col[32];
+        phase_map_columns(ph, k, rb, col);
+        ph.mapped = 0;
+        for (int l = 0; l < k; l++) ph.mapped |= col[l] != (1u << l) ? 1u : 0u;
+        part.mapped |= ph.mapped;
+    }
+    uint32_t fin[32];
+    for (int l = 0; l < k; l++) {
+        fin[l] = map_apply(N, k, end_col[l]);
+        part.mapped |= fin[l] != (1u << l) ? 1u : 0u;
+    }
+    for (int l = 0; l < 14; l++) part.fcol[l] = 0;
+    for (int j = 0; j < 16; j++) part.fj[j] = 0;
+    if (part.mapped) {
+        for (int l = 0; l < k; l++) part.fcol[l] = uint16_t(fin[l]);
+        for (uint32_t j = 0; j < (1u << rb); j++) part.fj[j] = uint16_t(map_apply(fin + (k - rb), rb, j));
+    }
+}
+
 void fold_leading_permutations(const Program& prog, PhaseInfo& pi)
 {
     for (int s = 0; s < kRegBits; s++) pi.gcol[s] = pi.scol[s];
@@ -1091,10 +1179,6 @@ void emit_standalone(Program& prog, const SOp& s)
     prog.steps.push_back(Step{Step::STANDALONE, uint32_t(prog.standalone.size())});
     prog.standalone.push_back(so);
 }
-
-// One-state sweeps of at most 2^kMapMaxTileBits1 tiles (twice the
-// workgroups a 256-CU device holds at once) use tile maps.
-constexpr int kMapMaxTileBits1 = 10;
 
 void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T, int n, int c,
                 const PlanOptions& opt)
@@ -1150,10 +1234,8 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     // two tile bits that are not both register bits of the phase taking it is
     // not run: after the phase, M becomes M o P (CX: mcol[c] ^= mcol[t]; SWAP:
     // two columns exchanged), and later phases and the tile's store address
-    // LDS through M.  One-state sweeps over many tiles stream at HBM speed,
-    // where the phases saved buy nothing and the mapped addresses cost LDS
-    // bank conflicts: those keep the plain layout.
-    const bool fold_map = opt.tile_map && (prog.n_states == 2 || (n - k) <= kMapMaxTileBits1);
+    // LDS through M.
+    const bool fold_map = opt.tile_map;
     uint32_t mcol[32];
     for (int l = 0; l < k; l++) mcol[l] = 1u << l;
     std::vector<Pending> remaining;
@@ -1398,8 +1480,15 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
     // final one, which may be the identity)
     for (int l = 0; l < k; l++) sw.mapped |= mcol[l] != (1u << l) ? 1u : 0u;
     for (uint32_t p = sw.phase_begin; p < sw.phase_end; p++) sw.mapped |= prog.phases[p].mapped;
-    if (sw.mapped)
+    if (sw.mapped) {
         for (int l = 0; l < k; l++) sw.fcol[l] = uint16_t(mcol[l]);
+        for (uint32_t j = 0; j < (1u << rb); j++) {   // M(j << (k - rb)): the store's register part
+            uint32_t m = 0;
+            for (int b = 0; b < rb; b++)
+                if ((j >> b) & 1u) m ^= mcol[k - rb + b];
+            sw.fj[j] = uint16_t(m);
+        }
+    }
     // qubits that select register-table variants become the tile id's high
     // bits: workgroups running at the same time then read the same variants
     for (size_t o = sw.op_begin; o < prog.ops.size(); o++)
@@ -1443,9 +1532,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
         return w;
     };
     sw.max_dense = dense_width(sw.op_begin, sw.op_end);
-    // (a sweep with a tile map stays whole: its parts would reload the tile
-    // in the plain layout)
-    if (prog.n_states == 1 && opt.coef_budget > 0 && sw.n_tiles >= 512 && !sw.mapped) {
+    if (prog.n_states == 1 && opt.coef_budget > 0 && sw.n_tiles >= 512 && sw.phase_end > sw.phase_begin) {
         // bytes a tile reads through the scalar cache per phase: every op
         // record (DOp stride) and the coefficients the op loads
         auto phase_bytes = [&](const PhaseInfo& ph) {
@@ -1479,6 +1566,13 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                 part.op_begin = prog.phases[p0].op_begin;
                 part.op_end = prog.phases[p - 1].op_end;
                 part.max_dense = dense_width(part.op_begin, part.op_end);
+                if (sw.mapped) {   // the map where the part ends: at phase p's start, or the final one
+                    uint32_t end_col[32];
+                    if (p < sw.phase_end) phase_map_columns(prog.phases[p], k, rb, end_col);
+                    else
+                        for (int l = 0; l < k; l++) end_col[l] = mcol[l];
+                    rebase_part(prog, part, rb, end_col);
+                }
                 prog.steps.push_back(Step{Step::SWEEP, uint32_t(prog.sweeps.size())});
                 prog.sweeps.push_back(part);
                 p0 = p;

@@ -86,11 +86,17 @@ def _state(n, gates, tile_map, state=None, tile=None):
 
 
 @pytest.mark.parametrize("n,name,tile", [(12, "random_layers", None), (20, "random_layers", None),
-                                         (21, "random_layers", 13), (21, "qft", None), (18, "qft", 8)])
+                                         (21, "random_layers", 13), (21, "qft", None), (18, "qft", 8),
+                                         (24, "random_layers", None), (24, "qft", None), (21, "hea", None)])
 def test_forward_tile_map_matches_oracle(n, name, tile):
+    # (21, "hea"): its first tile set is cut into two sweeps by the coefficient
+    # budget, each reloading the tile in the plain layout with rebased maps
     state = None
     if name == "random_layers":
         gates = W.random_layers(n=n, layers=12)
+    elif name == "hea":
+        gates = W.hea_ising(n=n, layers=8)[0]
+        state = W.random_state(n, seed=n)
     else:
         gates = W.qft(n)
         state = W.random_state(n, seed=n)

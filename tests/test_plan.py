@@ -132,6 +132,20 @@ def test_one_state_tile_map_matches_oracle(n, tile, low):
     assert _err(svmi.emulate(n, gates, st, tile_qubits=tile, low_qubits=low), O.run(n, gates, st)) < 1e-12
 
 
+def test_tile_map_across_coefficient_budget_parts():
+    # a sweep cut into parts by the coefficient budget reloads its tile in the
+    # plain layout at every part, so each part's maps are rebased (HEA-21: the
+    # first tile set is cut in two, both parts mapped)
+    n = 21
+    gates = W.hea_ising(n=n, layers=8)[0]
+    _, desc = svmi.plan(n, gates)
+    sweeps = [l for l in desc.splitlines() if l.strip().startswith("sweep")]
+    assert sweeps[0].split("ext=")[1] == sweeps[1].split("ext=")[1]
+    assert "[m]" in sweeps[0] and "[m]" in sweeps[1]
+    st = W.random_state(n, seed=5)
+    assert _err(svmi.emulate(n, gates, st), O.run(n, gates, st)) < 1e-12
+
+
 def test_tile_map_folds_the_hea_ladder():
     # the HEA backward program's CX ladder inside a tile costs no phase
     gates, slots, P, _ = W.hea_ising(n=20, layers=8)
