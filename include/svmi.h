@@ -116,9 +116,10 @@ typedef enum svmi_option {
                                  persistent with two LDS tile buffers, the next tile streaming
                                  in by LDS DMA during the current one's phases (one workgroup
                                  per CU); 0 = one workgroup per tile, two per CU (default)   */
-    SVMI_OPT_TILE_MAP = 11    /* adjoint sweeps: 1 = CX and SWAP gates on two tile qubits that a
-                                 phase does not hold in registers relabel the tile's LDS
-                                 indices instead of needing a phase (default); 0 = off       */
+    SVMI_OPT_TILE_MAP = 11    /* 1 = CX and SWAP gates on two tile qubits that a phase does not
+                                 hold in registers relabel the tile's LDS indices instead of
+                                 needing a phase (default; not with SVMI_OPT_SPECIALIZE);
+                                 0 = off                                                     */
 } svmi_option;
 
 /* ---- lifecycle: Simulation::Simulation / ~Simulation (svsim_amdgpu_sin.hpp:207-254,
