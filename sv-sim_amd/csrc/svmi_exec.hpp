@@ -389,6 +389,12 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_FENCE(DOP_LIFT3 + 100);
         return;
     }
+    if (h.type == DOP_PHASETAB) {
+        SVMI_FENCE(DOP_PHASETAB + 100);
+        apply_phasetab<N>(h, a, coef, grp, xf);
+        SVMI_FENCE(DOP_PHASETAB + 100);
+        return;
+    }
     switch (h.type) {
         SVMI_CASE_ID(DOP_DENSE, DOP_DENSE + 100, (dispatch_dense<kMaxQ, N>(h.smask, a, coef + h.coef)))
         SVMI_CASE_ID(DOP_X, DOP_X + 100, dispatch_x<N>(h.s0, a))
@@ -399,7 +405,6 @@ SVMI_HD inline void exec_state_op(const DOpHdr& h, const DOp* __restrict__ op, c
         SVMI_CASE_ID(DOP_DIAG, DOP_DIAG + 100, apply_diag<N>(*op, a, coef, goff, off, base))
         SVMI_CASE_ID(DOP_PHASESET, DOP_PHASESET + 100, apply_phaseset<N>(h, a, coef, goff, base, xf))
         SVMI_CASE_ID(DOP_CDENSE, DOP_CDENSE + 100, dispatch_cdense<N>(h, a, coef, goff, base))
-        SVMI_CASE_ID(DOP_PHASETAB, DOP_PHASETAB + 100, apply_phasetab<N>(h, a, coef, grp, xf))
     default: break;
     }
 }
