@@ -1113,15 +1113,22 @@ void map_inverse(const uint32_t* col, int k, uint32_t* inv)
     }
 }
 
-// A sweep cut into parts reloads the tile in the plain layout at every part:
-// the part's phases address LDS through N o M_p, N the inverse of the map at
-// the part's first phase, and its store through N o (map at the part's end).
-void rebase_part(Program& prog, SweepInfo& part, int rb, const uint32_t* end_col)
+// A sweep cut into parts reloads the tile in the plain layout at every part
+// after the first: the part's phases address LDS through N o M_p, N the
+// inverse of the map at the part's first phase, and its store through
+// N o (map at the part's end).  The first part loads the sweep's input, to
+// which the maps already refer (N = identity; folds made before the sweep's
+// first phase stay in its map).
+void rebase_part(Program& prog, SweepInfo& part, int rb, bool first, const uint32_t* end_col)
 {
     const int k = int(part.k);
     uint32_t start[32], N[32];
-    phase_map_columns(prog.phases[part.phase_begin], k, rb, start);
-    map_inverse(start, k, N);
+    if (first) {
+        for (int l = 0; l < k; l++) N[l] = 1u << l;
+    } else {
+        phase_map_columns(prog.phases[part.phase_begin], k, rb, start);
+        map_inverse(start, k, N);
+    }
     part.mapped = 0;
     for (uint32_t p = part.phase_begin; p < part.phase_end; p++) {
         PhaseInfo& ph = prog.phases[p];
@@ -1571,7 +1578,7 @@ void emit_sweep(Program& prog, const std::vector<const SOp*>& taken, uint64_t T,
                     if (p < sw.phase_end) phase_map_columns(prog.phases[p], k, rb, end_col);
                     else
                         for (int l = 0; l < k; l++) end_col[l] = mcol[l];
-                    rebase_part(prog, part, rb, end_col);
+                    rebase_part(prog, part, rb, p0 == sw.phase_begin, end_col);
                 }
                 prog.steps.push_back(Step{Step::SWEEP, uint32_t(prog.sweeps.size())});
                 prog.sweeps.push_back(part);
